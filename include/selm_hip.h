@@ -1,0 +1,194 @@
+/*
+ * selm_hip.h -- C ABI of the MI355X-native SELM fluctuating-hydrodynamics step.
+ *
+ * The reference couples a SELM step to LAMMPS through two C++ plugin classes
+ * that share raw device pointers (Eulerian_Period: cuda/SELM_Eulerian_Period.cu,
+ * Lagrangian_LAMMPS: cuda/SELM_Lagrangian_LAMMPS.cu, driven by Selm:
+ * cuda/SELM.cu; prefix cuda/ = data/src/selm-main/src/cuda/).  This header is
+ * the boundary that replaces them: plain pointers and sizes, int status codes,
+ * no exceptions, no C++ or torch types.  One selm_ctx is one Selm system
+ * (one periodic grid + one bead set); it owns all its device memory.  A
+ * selm_ctx is thread-compatible (one host thread at a time); there is no
+ * global state.
+ *
+ * Arrays of beads are AoS [M][3] doubles, exactly LAMMPS's atom->x/v/f
+ * layout.  Grid fields are [z][y][x] with x fastest; component d of the
+ * velocity lives on the MAC face (i, j+1/2, k+1/2) dx for d = 0, etc.
+ */
+#ifndef SELM_HIP_H
+#define SELM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (0 = OK); text via selm_last_error / selm_pool_last_error */
+#define SELM_OK 0
+#define SELM_ERR_ARG 1          /* invalid argument (the reference would throw or crash) */
+#define SELM_ERR_HIP 2          /* HIP runtime error (reference: CHECK_RUNTIME -> exit(-1), cuda/check.cuh:6-10) */
+#define SELM_ERR_STATE 3        /* call out of order (e.g. step before init) */
+#define SELM_ERR_NOMEM 4        /* device allocation failed */
+#define SELM_ERR_UNSUPPORTED 5  /* grid edge not a product of 2,3,5 in [4, 512] */
+#define SELM_ERR_POOL_FULL 6    /* pool: no slot or not enough budget for this footprint */
+
+/* precision of the bulk (grid, FFT, spread, interpolation) arithmetic; bead
+ * positions, velocities and forces are FP64 in both */
+#define SELM_PREC_FP64 0
+#define SELM_PREC_FP32_BULK 1
+
+/* per-stage indices for selm_stage_times */
+#define SELM_NUM_STAGES 10
+
+typedef struct selm_ctx selm_ctx;
+typedef struct selm_pool selm_pool;
+
+/* Inputs of one system.  Maps the reference's input surface:
+ *   dims ........ Model.SELM numMeshPtsPerDir (cuda/SELM.cu:44)
+ *   lo, hi ...... LAMMPS domain->boxlo/boxhi (cuda/SELM_Lagrangian_LAMMPS.cu:62-63);
+ *                 like the reference, dx = (hi[0]-lo[0])/dims[0] and the bead
+ *                 wrap uses lo[0]/hi[0] for every dimension
+ *   rho mu kB T . Model.SELM rho, mu, KB, T (cuda/SELM.cu:20-24)
+ *   dt .......... LAMMPS `timestep` (cuda/SELM.cu:37)
+ *   seed ........ Model.SELM SELM_Seed (cuda/SELM.cu:29)
+ *   stochastic .. Model.SELM flagStochasticDriving (cuda/SELM.cu:28)
+ *   precision ... SELM_PREC_* (new; default FP64 keeps the reference's arithmetic)
+ *   f4_quirk .... 1 (default): the reference's Peskin factor, which stays 1.0 at
+ *                 a stencil distance r >= 2 (cuda/SELM_Kernel.cu:387-395);
+ *                 0: the textbook kernel (factor 0 there)                     */
+typedef struct selm_params {
+    int dims[3];
+    double lo[3], hi[3];
+    double rho, mu, kB, T, dt;
+    uint64_t seed;
+    int stochastic;
+    int precision;
+    int f4_quirk;
+    int reserved;
+} selm_params;
+
+/* Fill p with the reference's defaults (SELM_Seed 123456, flagStochasticDriving 0:
+ * cuda/SELM.cuh:13-15) and FP64, f4_quirk = 1. */
+int selm_params_default(selm_params *p);
+
+/* Eulerian_Period::initial's allocations (cuda/SELM_Eulerian_Period.cu:14-76)
+ * plus the bead-side state; replaces `new Selm` + `new Eulerian_Period`
+ * (cuda/SELM.cu:17-48). */
+int selm_create(const selm_params *p, selm_ctx **out);
+
+/* Selm::final + ~Selm (cuda/SELM.cu:50-63): frees everything the ctx owns. */
+int selm_destroy(selm_ctx *ctx);
+
+/* Text of the last error on this ctx ("" if none). */
+const char *selm_last_error(const selm_ctx *ctx);
+
+/* Bead state as LAMMPS holds it after read_data (Lagrangian_LAMMPS::initial,
+ * cuda/SELM_Lagrangian_LAMMPS.cu:53-69,84-92): x, v AoS [M][3] (v may be NULL
+ * for zero velocities, the read_data default), m [M] per-bead mass. Copied in. */
+int selm_set_beads(selm_ctx *ctx, int M, const double *x, const double *v, const double *m);
+
+/* LAMMPS `bond_style harmonic` bonds (Bonds section + bond_coeff), evaluated
+ * where the reference calls integrate->setup_minimal
+ * (cuda/SELM_Lagrangian_LAMMPS.cu:122): E = K (r - r0)^2 with the minimum image
+ * convention.  ij [nb][2] zero-based bead indices, K [nb], r0 [nb]. */
+int selm_set_bonds(selm_ctx *ctx, int nb, const int *ij, const double *K, const double *r0);
+
+/* LAMMPS `fix addforce` with a constant vector on the beads idx[n]: F [n][3]. */
+int selm_set_external_force(selm_ctx *ctx, int n, const int *idx, const double *F);
+
+/* Host-force hook: overwrite the conservative force for the next step, the
+ * reference's H2D copy of LAMMPS forces (cuda/SELM_Lagrangian_LAMMPS.cu:129).
+ * Forces from selm_set_bonds/external are recomputed after every step. */
+int selm_set_conservative_force(selm_ctx *ctx, const double *f);
+
+/* Selm::initial (cuda/SELM.cu:55-58): LAMMPS setup(1) wraps x into [lo, hi)
+ * and computes the forces at x^0; the fluid starts at u = 0; step counter 0. */
+int selm_init(selm_ctx *ctx);
+
+/* nsteps iterations of Selm::run's loop body (cuda/SELM.cu:73-90): Lagrangian
+ * update with u^n, fluid update, new thermal field and forces at x^{n+1}.
+ * Enqueued on `stream` (a hipStream_t; NULL = the ctx's own stream) as one
+ * captured hipGraph per step; returns without waiting. */
+int selm_step(selm_ctx *ctx, int nsteps, void *stream);
+
+/* Wait for all work enqueued by this ctx. */
+int selm_sync(selm_ctx *ctx);
+
+/* Bead state after the last step (waits for it): x, v as in LAMMPS;
+ * f_cons = conservative force at the current x (what atom->f holds after
+ * setup_minimal); f_total = (drag + thermal + dt f_cons)/dt of the last update
+ * (what d_f holds after lagrangian_update and particleF.dat records,
+ * cuda/SELM_Kernel.cu:357-358).  Any pointer may be NULL. */
+int selm_get_beads(selm_ctx *ctx, double *x, double *v, double *f_cons, double *f_total);
+
+/* Velocity component comp (0..2) of u^n as N doubles [z][y][x]
+ * (the reference's d_u[cur_u], cuda/SELM_Eulerian.cuh:15-17). */
+int selm_get_field(selm_ctx *ctx, int comp, double *u);
+
+/* Overwrite component comp of u^n (N doubles); for restarts and tests. */
+int selm_set_field(selm_ctx *ctx, int comp, const double *u);
+
+/* Number of completed steps (Selm::timestep, cuda/SELM.cuh:11). */
+long long selm_timestep(selm_ctx *ctx);
+
+/* Device bytes one system with these parameters, M beads and nb bonds holds. */
+int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
+
+/* Run nsteps steps stage by stage with HIP events around each stage on the
+ * ctx stream; ms[s] receives the summed milliseconds of stage s
+ * (0 bead update, 1 binning, 2 spread, 3 viscous u*, 4 FFT x, 5 FFT y,
+ *  6 FFT z + projection, 7 inverse FFT y, 8 inverse FFT x, 9 bond forces). */
+int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
+
+/* Sum over the grid of u* for each component before the k=0 mode is
+ * removed in the last step (FP64): rho * dx^3 * this equals the fluid
+ * momentum injected by the beads' drag and thermal kicks. */
+int selm_mean_mode(selm_ctx *ctx, double out[3]);
+
+/* The ctx's own stream (hipStream_t). */
+void *selm_stream(selm_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Ensembles and serving (no counterpart in the reference, which runs one
+ * system per process: cuda/main.cu:5-12).  A pool owns one device arena of
+ * hbm_budget bytes; systems are admitted by their footprint (selm_footprint)
+ * into free space, stepped together by one batched launch per stage
+ * (captured as one hipGraph per pool composition), and retired to free their
+ * space.  Each system's result is independent of what else is in the pool.
+ * ------------------------------------------------------------------------- */
+typedef struct selm_pool_params {
+    size_t hbm_budget;   /* bytes of the device arena */
+    int max_replicas;    /* slots */
+    int precision;       /* SELM_PREC_*; all systems in a pool share it */
+    int use_graph;       /* 1: replay a captured hipGraph per step */
+} selm_pool_params;
+
+int selm_pool_create(const selm_pool_params *pp, selm_pool **out);
+int selm_pool_destroy(selm_pool *pool);
+const char *selm_pool_last_error(const selm_pool *pool);
+
+/* Admit one system and run its Selm::initial.  p->precision is ignored (the
+ * pool's precision applies).  Returns SELM_ERR_POOL_FULL if no slot or no
+ * contiguous free space of its footprint is available. */
+int selm_pool_admit(selm_pool *pool, const selm_params *p, int M, const double *x,
+                    const double *v, const double *m, int nb, const int *ij, const double *K,
+                    const double *r0, int next, const int *idx, const double *F, int *slot);
+
+/* Advance every admitted system by nsteps steps; returns after enqueueing. */
+int selm_pool_step(selm_pool *pool, int nsteps);
+int selm_pool_sync(selm_pool *pool);
+int selm_pool_get_beads(selm_pool *pool, int slot, double *x, double *v, double *f_cons,
+                        double *f_total);
+int selm_pool_get_field(selm_pool *pool, int slot, int comp, double *u);
+long long selm_pool_timestep(selm_pool *pool, int slot);
+int selm_pool_retire(selm_pool *pool, int slot);
+/* used/budget bytes of the arena, number of admitted systems */
+int selm_pool_stats(selm_pool *pool, size_t *used, size_t *budget, int *active);
+void *selm_pool_stream(selm_pool *pool);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SELM_HIP_H */

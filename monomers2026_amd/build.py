@@ -1,0 +1,61 @@
+"""Build recipe for libselm_hip.so (gfx950) and the C++ driver executable.
+
+Everything is compiled in-tree with hipcc; no JIT caches.  ``python -m
+monomers2026_amd.build`` rebuilds from scratch.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libselm_hip.so")
+DRIVER = os.path.join(PKG, "selm_hip_driver")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# Packed FP32 instructions are disabled for all device code.
+DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-Wno-unused-command-line-argument"]
+
+LIB_SOURCES = ["selm_kernels.hip", "selm_engine.cpp"]
+DRIVER_SOURCES = ["host/selm_main.cpp", "host/SELM.cpp", "host/SELM_Eulerian_Period_HIP.cpp",
+                  "host/SELM_Lagrangian_Chain_HIP.cpp", "host/lammps_subset.cpp", "host/dcd_writer.cpp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=CSRC)
+
+
+def build_lib(verbose=False):
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + DEVICE_FLAGS + ["-shared", "-I", INCLUDE, "-I", CSRC, "-o", tmp] + LIB_SOURCES
+    _run(cmd, verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_driver(verbose=False):
+    srcs = [s for s in DRIVER_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    if not srcs:
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", os.path.join(CSRC, "host"),
+           "-o", DRIVER] + srcs + ["-L", PKG, "-lselm_hip", "-Wl,-rpath,$ORIGIN",
+                                   "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    _run(cmd, verbose)
+    return DRIVER
+
+
+def build(verbose=False):
+    build_lib(verbose)
+    build_driver(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

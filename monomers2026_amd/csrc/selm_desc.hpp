@@ -1,0 +1,63 @@
+// selm_desc.hpp -- the per-replica descriptor read by every kernel, and the
+// per-stage work tables that let one launch cover many heterogeneous replicas.
+#pragma once
+
+#include <stdint.h>
+
+#include "selm_device.hpp"
+
+namespace selm {
+
+// Fluid arrays of one replica (T = double or float):
+//   u     [3][nz][ny][nx]   velocity u^n (MAC faces: u_x at (i, j+1/2, k+1/2) dx)
+//   fsp   [3][nz][ny][nx]   spread impulse, then u* in place
+//   spec  [3][nz][ny][xc]   half spectrum (xc = nx/2+1), unnormalised DFT
+// Bead arrays (always FP64, AoS [M][3] like LAMMPS atom->x):
+//   x v fc ftot fext tf sg ; m [M]
+struct RepDev {
+    // ---- grid
+    int n[3];             // nx, ny, nz
+    int xc;               // nx/2 + 1
+    long long N, Nc;      // nx*ny*nz, xc*ny*nz
+    void *u, *fsp, *spec;
+    const void *tw[3];    // cplx<T>[n_a]: e^{-2 pi i m / n_a}
+    const void *ktab[3];  // per axis, T[4*n_a]: {lam_a, Dre_a, Dim_a, sqrt-part}
+    RadixPlan plan[3];
+    int bx, by, bz;       // rows / columns per workgroup in the X, Y, Z passes
+    // ---- tiles for binning / spread
+    int te[3];            // tile edge per axis (cells)
+    int tn[3];            // tiles per axis
+    int ntiles;
+    // ---- beads
+    int M;
+    double *x, *v, *fc, *ftot, *m, *fext, *tf, *sg;
+    int *key, *tcount, *tstart, *tcursor, *permu, *perm;
+    int *adj_ptr, *adj_i1, *adj_i2;   // CSR over beads: bonds touching each bead, bond order
+    double *adj_K, *adj_r0;
+    // ---- scalars
+    double dx, lo, hi, L;      // lo[0], hi[0] (reference wraps and grids with dim 0 only)
+    double Lb[3];              // per-dimension box lengths (bond minimum image)
+    double dt, rho, mu;
+    double ups;                // 6 pi (2 dx) mu
+    double bead_coef;          // sqrt(2 kB T ups dt)
+    double noise_amp;          // N * sqrt(2 kB T dt mu / N) / rho
+    double inv_dxsq;
+    uint64_t seed;
+    int noise, f4;
+    long long *ts;             // completed steps (device counter)
+    double *diag;              // [3] k=0 mode of w (momentum diagnostic), FP64
+};
+
+// Work table for one stage: item g of the launch belongs to replica r with
+// start[r] <= g < start[r+1].
+__device__ __forceinline__ int find_rep(const long long *start, int nrep, long long g) {
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace selm

@@ -1,0 +1,300 @@
+// selm_device.hpp -- device-side building blocks for the SELM step on CDNA4.
+//
+// Complex arithmetic, the Philox4x32-10 counter-based generator, the Peskin
+// 4-point 1-D factor, radix-2/3/4/5/8 butterflies and a workgroup-cooperative
+// Stockham FFT over LDS-resident batches of sequences.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace selm {
+
+template <typename T>
+struct cplx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mk(T a, T b) {
+    cplx<T> r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return mk<T>(a.x, -a.y); }
+// multiply by +i (dir>0) or -i (dir<0)
+template <typename T>
+__device__ __forceinline__ cplx<T> mul_i(cplx<T> a, int dir) {
+    return dir > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  Counter = {index, step, slot, stream}, key = seed.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = M0 * c[0], hi0 = __umulhi(M0, c[0]);
+        uint32_t lo1 = M1 * c[2], hi1 = __umulhi(M1, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
+constexpr uint32_t kStreamFluid = 0u;
+constexpr uint32_t kStreamBead = 1u;
+
+// Box-Muller pair in FP64.  u1 in (0,1] from 53 bits of words 0,1; angle
+// 2*pi*t with t in [-1/2,1/2) from 53 bits of words 2,3.
+__device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint32_t stream, uint32_t step,
+                                                uint32_t index, uint32_t slot, double &z0,
+                                                double &z1) {
+    uint32_t c[4] = {index, step, slot, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
+    uint64_t b = (((uint64_t)c[2] << 32) | c[3]) >> 11;
+    double u1 = (double)(a + 1) * 0x1p-53;
+    int64_t bc = (int64_t)b - ((int64_t)1 << 52);
+    double t = (double)bc * 0x1p-53;
+    double rad = sqrt(-2.0 * log(u1));
+    double th = 2.0 * 3.14159265358979323846 * t;
+    double s, co;
+    sincos(th, &s, &co);
+    z0 = rad * co;
+    z1 = rad * s;
+}
+
+// The same pair evaluated in FP32 from the same 53-bit integers.  -ln(u1) is
+// taken from u1 when u1 < 1/2 and from log1p(-(1-u1)) otherwise, with 1-u1
+// formed exactly in integer arithmetic, so neither branch loses the low bits
+// of the uniform to rounding.
+__device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint32_t stream, uint32_t step,
+                                                uint32_t index, uint32_t slot, float &z0,
+                                                float &z1) {
+    uint32_t c[4] = {index, step, slot, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
+    uint64_t b = (((uint64_t)c[2] << 32) | c[3]) >> 11;
+    float mlog;
+    if (a + 1 < ((uint64_t)1 << 52)) {
+        mlog = -logf((float)(a + 1) * 0x1p-53f);
+    } else {
+        uint64_t om = ((uint64_t)1 << 53) - (a + 1);  // (1-u1) * 2^53, exact
+        mlog = -log1pf(-(float)om * 0x1p-53f);
+    }
+    int64_t bc = (int64_t)b - ((int64_t)1 << 52);
+    float t = (float)bc * 0x1p-53f;
+    float rad = sqrtf(2.0f * mlog);
+    float s, co;
+    sincospif(2.0f * t, &s, &co);
+    z0 = rad * co;
+    z1 = rad * s;
+}
+
+template <typename T>
+struct Normal;
+template <>
+struct Normal<double> {
+    __device__ static void pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t idx,
+                                uint32_t slot, double &a, double &b) {
+        normal_pair_f64(seed, stream, step, idx, slot, a, b);
+    }
+};
+template <>
+struct Normal<float> {
+    __device__ static void pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t idx,
+                                uint32_t slot, float &a, float &b) {
+        normal_pair_f32(seed, stream, step, idx, slot, a, b);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Peskin 4-point 1-D factor.  Branch structure of cuda/SELM_Kernel.cu:387-395:
+// the factor is (3-2r+sqrt(1+4r(1-r)))/8 for r<1, (5-2r-sqrt(-7+4r(3-r)))/8
+// for 1<=r<2, and otherwise stays at 1.0 (f4 != 0, the reference) or 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double peskin_phi(double r, int f4) {
+    if (r < 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r * (1.0 - r))) / 8.0;
+    if (r < 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 4.0 * r * (3.0 - r))) / 8.0;
+    return f4 ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ int wrap_idx(int p, int n) {
+    int q = p % n;
+    return q < 0 ? q + n : q;
+}
+
+// ---------------------------------------------------------------------------
+// Radix-R DFTs in registers.  dir = -1: forward (e^{-i}), +1: inverse.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void dft2(cplx<T> *v) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T> &v0, cplx<T> &v1, cplx<T> &v2, cplx<T> &v3, int dir) {
+    cplx<T> t0 = cadd(v0, v2), t1 = csub(v0, v2);
+    cplx<T> t2 = cadd(v1, v3), t3 = mul_i(csub(v1, v3), dir);
+    v0 = cadd(t0, t2);
+    v2 = csub(t0, t2);
+    v1 = cadd(t1, t3);
+    v3 = csub(t1, t3);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(cplx<T> *v, int dir) {
+    const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+    cplx<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    cplx<T> x0 = cadd(v[0], s);
+    cplx<T> t = mk<T>(v[0].x - (T)0.5 * s.x, v[0].y - (T)0.5 * s.y);
+    cplx<T> u = mul_i(cscale(d, h), dir);
+    v[0] = x0;
+    v[1] = cadd(t, u);
+    v[2] = csub(t, u);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(cplx<T> *v, int dir) {
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    cplx<T> a1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+    cplx<T> a2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    cplx<T> x0 = cadd(v[0], cadd(a1, a2));
+    cplx<T> r1 = mk<T>(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    cplx<T> r2 = mk<T>(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    cplx<T> i1 = mul_i(mk<T>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y), dir);
+    cplx<T> i2 = mul_i(mk<T>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y), dir);
+    v[0] = x0;
+    v[1] = cadd(r1, i1);
+    v[4] = csub(r1, i1);
+    v[2] = cadd(r2, i2);
+    v[3] = csub(r2, i2);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(cplx<T> *v, int dir) {
+    const T c = (T)0.70710678118654752440;
+    cplx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cplx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3, dir);
+    dft4(o0, o1, o2, o3, dir);
+    // W8^k, W8 = e^{dir*2*pi*i/8}
+    cplx<T> w1 = mk<T>(c, (T)dir * c);
+    cplx<T> w3 = mk<T>(-c, (T)dir * c);
+    o1 = cmul(o1, w1);
+    o2 = mul_i(o2, dir);
+    o3 = cmul(o3, w3);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+// Radix plan for one length: stage radices in execution order.
+struct RadixPlan {
+    int n;
+    int nst;
+    int r[12];
+};
+
+// One Stockham stage over S interleaved sequences held element-major in LDS:
+// element e of sequence s lives at buf[e*S + s].  Reads src, writes dst.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src,
+                                               cplx<T> *__restrict__ dst, int n, int S, int Ns,
+                                               const cplx<T> *__restrict__ tw, int dir, int tid,
+                                               int nthr) {
+    const int nb = n / R;
+    const int total = S * nb;
+    const int twstep = n / (Ns * R);
+    for (int t = tid; t < total; t += nthr) {
+        int s = t % S;
+        int j = t / S;
+        int k = j % Ns;
+        cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = src[(j + r * nb) * S + s];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                cplx<T> w = tw[r * k * twstep];
+                if (dir > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        if constexpr (R == 2) dft2(v);
+        else if constexpr (R == 3) dft3(v, dir);
+        else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3], dir);
+        else if constexpr (R == 5) dft5(v, dir);
+        else dft8(v, dir);
+        int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) dst[(base + r * Ns) * S + s] = v[r];
+    }
+}
+
+// Full FFT of S sequences of length plan.n in LDS (ping-pong between a and b).
+// tw[m] = e^{-2 pi i m / n}.  Returns the buffer holding the result.  The
+// caller must __syncthreads() before (data loaded) and uses the result after
+// this function's final barrier.
+template <typename T>
+__device__ cplx<T> *lds_fft(cplx<T> *a, cplx<T> *b, const RadixPlan &plan, int S,
+                            const cplx<T> *tw, int dir) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    int Ns = 1;
+    cplx<T> *src = a, *dst = b;
+    for (int st = 0; st < plan.nst; st++) {
+        int R = plan.r[st];
+        switch (R) {
+            case 2: stockham_stage<T, 2>(src, dst, plan.n, S, Ns, tw, dir, tid, nthr); break;
+            case 3: stockham_stage<T, 3>(src, dst, plan.n, S, Ns, tw, dir, tid, nthr); break;
+            case 4: stockham_stage<T, 4>(src, dst, plan.n, S, Ns, tw, dir, tid, nthr); break;
+            case 5: stockham_stage<T, 5>(src, dst, plan.n, S, Ns, tw, dir, tid, nthr); break;
+            default: stockham_stage<T, 8>(src, dst, plan.n, S, Ns, tw, dir, tid, nthr); break;
+        }
+        __syncthreads();
+        Ns *= R;
+        cplx<T> *t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+// Wave-wide (64 lanes) sum.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+}  // namespace selm

@@ -1,0 +1,1078 @@
+// selm_engine.cpp -- host side of the C ABI (include/selm_hip.h): replica
+// geometry and memory layout, the batched step (work tables, hipGraph), and
+// the single-system (selm_ctx) and pool (selm_pool) entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "selm_hip.h"
+#include "selm_kernels.hpp"
+
+namespace selm {
+
+constexpr size_t kLdsMax = 65536;
+constexpr double kPi = 3.14159265358979323846;
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            err = std::string(#expr) + " failed: " + hipGetErrorString(e_);           \
+            return SELM_ERR_HIP;                                                      \
+        }                                                                             \
+    } while (0)
+
+static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---------------------------------------------------------------------------
+// Geometry of one grid
+// ---------------------------------------------------------------------------
+struct Geometry {
+    int n[3];
+    int xc;
+    long long N, Nc;
+    RadixPlan plan[3];
+    int te[3], tn[3], ntiles;
+    int bx, by, bz;
+};
+
+static bool make_plan(int n, RadixPlan &p) {
+    p.n = n;
+    p.nst = 0;
+    int m = n;
+    while (m % 8 == 0 && p.nst < 12) { p.r[p.nst++] = 8; m /= 8; }
+    while (m % 4 == 0 && p.nst < 12) { p.r[p.nst++] = 4; m /= 4; }
+    while (m % 2 == 0 && p.nst < 12) { p.r[p.nst++] = 2; m /= 2; }
+    while (m % 3 == 0 && p.nst < 12) { p.r[p.nst++] = 3; m /= 3; }
+    while (m % 5 == 0 && p.nst < 12) { p.r[p.nst++] = 5; m /= 5; }
+    return m == 1;
+}
+
+static size_t lds_x(const Geometry &G, int twmax, size_t csz) {
+    int S = (G.bx / 2) | 1;
+    return ((size_t)twmax + 2 * (size_t)(G.n[0] + 2) * S) * csz;
+}
+static size_t lds_y(const Geometry &G, int twmax, size_t csz) {
+    return ((size_t)twmax + 2 * (size_t)G.n[1] * G.by) * csz;
+}
+static size_t lds_z(const Geometry &G, int twmax, size_t csz) {
+    return ((size_t)twmax + 2 * (size_t)G.n[2] * 3 * G.bz) * csz;
+}
+
+static int make_geometry(const selm_params &p, bool fp64, Geometry &G, std::string &err) {
+    for (int a = 0; a < 3; a++) {
+        int n = p.dims[a];
+        if (n < 4 || n > 512) {
+            err = "grid edge " + std::to_string(n) + " outside [4, 512]";
+            return SELM_ERR_UNSUPPORTED;
+        }
+        G.n[a] = n;
+        if (!make_plan(n, G.plan[a])) {
+            err = "grid edge " + std::to_string(n) + " is not a product of 2, 3 and 5";
+            return SELM_ERR_UNSUPPORTED;
+        }
+        // tile edge: <= 8 cells, ragged last tile of 0 or >= 2 cells so a
+        // stencil (reach 2) can never skip over a whole tile
+        int te = n;
+        if (n > 8) {
+            te = 8;
+            for (int c : {8, 7, 6, 5, 4})
+                if (n % c != 1) { te = c; break; }
+        }
+        G.te[a] = te;
+        G.tn[a] = (n + te - 1) / te;
+    }
+    G.xc = G.n[0] / 2 + 1;
+    G.N = (long long)G.n[0] * G.n[1] * G.n[2];
+    G.Nc = (long long)G.xc * G.n[1] * G.n[2];
+    G.ntiles = G.tn[0] * G.tn[1] * G.tn[2];
+    const size_t csz = fp64 ? 16 : 8;
+    G.bx = fp64 ? 16 : 32;
+    G.by = fp64 ? 8 : 16;
+    G.bz = fp64 ? 4 : 8;
+    const int twmax = 512;
+    while (lds_x(G, twmax, csz) > kLdsMax && G.bx > 2) G.bx /= 2;
+    while (lds_y(G, twmax, csz) > kLdsMax && G.by > 1) G.by /= 2;
+    while (lds_z(G, twmax, csz) > kLdsMax && G.bz > 1) G.bz /= 2;
+    if (lds_x(G, twmax, csz) > kLdsMax || lds_y(G, twmax, csz) > kLdsMax || lds_z(G, twmax, csz) > kLdsMax) {
+        err = "grid too large for the LDS budget";
+        return SELM_ERR_UNSUPPORTED;
+    }
+    return SELM_OK;
+}
+
+static size_t grid_bytes(const Geometry &G, bool fp64) {
+    size_t s = fp64 ? 8 : 4;
+    size_t b = 0;
+    b += 2 * al(3 * (size_t)G.N * s);     // u, fsp
+    b += al(3 * (size_t)G.Nc * 2 * s);    // spec
+    for (int a = 0; a < 3; a++) b += al((size_t)G.n[a] * 2 * s) + al(4 * (size_t)G.n[a] * s);
+    b += al(8) + al(3 * 8);               // ts, diag
+    b += 2 * al((size_t)G.ntiles * 4) + al(((size_t)G.ntiles + 1) * 4);
+    return b;
+}
+
+static size_t bead_bytes(int M, int nadj) {
+    size_t b = 0;
+    b += 7 * al(3 * (size_t)M * 8);
+    b += al((size_t)M * 8);
+    b += 3 * al((size_t)M * 4);
+    b += al(((size_t)M + 1) * 4);
+    b += 2 * al((size_t)nadj * 4) + 2 * al((size_t)nadj * 8);
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// One replica (one Selm system)
+// ---------------------------------------------------------------------------
+struct Replica {
+    selm_params P{};
+    bool fp64 = true;
+    Geometry G{};
+    RepDev D{};
+    int M = 0, nb = 0;
+    char *grid_base = nullptr;
+    char *bead_base = nullptr;
+    size_t grid_sz = 0, bead_sz = 0;
+    std::vector<double> hx, hv, hm, hfext;
+    std::vector<int> adj_ptr, adj_i1, adj_i2;
+    std::vector<double> adj_K, adj_r0;
+    bool inited = false;
+};
+
+static void carve_grid(Replica &R, char *base) {
+    const Geometry &G = R.G;
+    size_t s = R.fp64 ? 8 : 4;
+    char *p = base;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += al(bytes);
+        return q;
+    };
+    R.grid_base = base;
+    R.D.u = take(3 * (size_t)G.N * s);
+    R.D.fsp = take(3 * (size_t)G.N * s);
+    R.D.spec = take(3 * (size_t)G.Nc * 2 * s);
+    for (int a = 0; a < 3; a++) {
+        R.D.tw[a] = take((size_t)G.n[a] * 2 * s);
+        R.D.ktab[a] = take(4 * (size_t)G.n[a] * s);
+    }
+    R.D.ts = (long long *)take(8);
+    R.D.diag = (double *)take(3 * 8);
+    R.D.tcount = (int *)take((size_t)G.ntiles * 4);
+    R.D.tcursor = (int *)take((size_t)G.ntiles * 4);
+    R.D.tstart = (int *)take(((size_t)G.ntiles + 1) * 4);
+}
+
+static void carve_beads(Replica &R, char *base) {
+    const int M = R.M;
+    const size_t nadj = R.adj_i1.size();
+    char *p = base;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += al(bytes);
+        return q;
+    };
+    R.bead_base = base;
+    R.D.x = (double *)take(3 * (size_t)M * 8);
+    R.D.v = (double *)take(3 * (size_t)M * 8);
+    R.D.fc = (double *)take(3 * (size_t)M * 8);
+    R.D.ftot = (double *)take(3 * (size_t)M * 8);
+    R.D.fext = (double *)take(3 * (size_t)M * 8);
+    R.D.tf = (double *)take(3 * (size_t)M * 8);
+    R.D.sg = (double *)take(3 * (size_t)M * 8);
+    R.D.m = (double *)take((size_t)M * 8);
+    R.D.key = (int *)take((size_t)M * 4);
+    R.D.permu = (int *)take((size_t)M * 4);
+    R.D.perm = (int *)take((size_t)M * 4);
+    R.D.adj_ptr = (int *)take(((size_t)M + 1) * 4);
+    R.D.adj_i1 = (int *)take(nadj * 4);
+    R.D.adj_i2 = (int *)take(nadj * 4);
+    R.D.adj_K = (double *)take(nadj * 8);
+    R.D.adj_r0 = (double *)take(nadj * 8);
+}
+
+static int replica_setup(Replica &R, const selm_params &p, bool fp64, std::string &err) {
+    R.P = p;
+    R.fp64 = fp64;
+    int rc = make_geometry(p, fp64, R.G, err);
+    if (rc) return rc;
+    if (!(p.hi[0] > p.lo[0]) || !(p.dt > 0) || !(p.rho > 0) || !(p.mu >= 0)) {
+        err = "invalid box, dt, rho or mu";
+        return SELM_ERR_ARG;
+    }
+    const Geometry &G = R.G;
+    RepDev &D = R.D;
+    std::memset(&D, 0, sizeof(D));
+    for (int a = 0; a < 3; a++) {
+        D.n[a] = G.n[a];
+        D.plan[a] = G.plan[a];
+        D.te[a] = G.te[a];
+        D.tn[a] = G.tn[a];
+        D.Lb[a] = p.hi[a] - p.lo[a];
+    }
+    D.xc = G.xc;
+    D.N = G.N;
+    D.Nc = G.Nc;
+    D.bx = G.bx;
+    D.by = G.by;
+    D.bz = G.bz;
+    D.ntiles = G.ntiles;
+    // dx = (hi[0]-lo[0])/dim[0] (cuda/SELM_Eulerian_Period.cu:18)
+    D.dx = (p.hi[0] - p.lo[0]) / p.dims[0];
+    D.lo = p.lo[0];
+    D.hi = p.hi[0];
+    D.L = p.hi[0] - p.lo[0];
+    D.dt = p.dt;
+    D.rho = p.rho;
+    D.mu = p.mu;
+    // cuda/SELM_Lagrangian_LAMMPS.cu:138-139
+    D.ups = 6.0 * kPi * 2 * D.dx * p.mu;
+    D.bead_coef = std::sqrt(2.0 * p.kB * p.T * D.ups * p.dt);
+    // cuda/SELM_Eulerian_Period.cu:141, times N (DFT of the c2r output) / rho (u* += f/rho)
+    D.noise_amp = (double)G.N * std::sqrt(2 * p.kB * p.T * p.dt * p.mu / (double)G.N) / p.rho;
+    D.inv_dxsq = 1.0 / (D.dx * D.dx);
+    D.seed = p.seed;
+    D.noise = p.stochastic ? 1 : 0;
+    D.f4 = p.f4_quirk ? 1 : 0;
+    return SELM_OK;
+}
+
+// Twiddle and wavenumber tables of one replica, uploaded once.
+static int upload_tables(Replica &R, hipStream_t st, std::string &err) {
+    const Geometry &G = R.G;
+    for (int a = 0; a < 3; a++) {
+        const int n = G.n[a];
+        std::vector<double> tw(2 * n), kt(4 * n);
+        for (int m = 0; m < n; m++) {
+            long double th = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+            tw[2 * m] = (double)cosl(th);
+            tw[2 * m + 1] = (double)(-sinl(th));
+            long double sh = sinl(th / 2);
+            if (R.fp64) {
+                // lambda term exactly as cuda/SELM_Kernel.cu:19
+                kt[4 * m] = 2 * (std::cos(2 * kPi * m / n) - 1);
+            } else {
+                kt[4 * m] = (double)(-4.0L * sh * sh);
+            }
+            kt[4 * m + 1] = (double)(-2.0L * sh * sh);  // Re(e^{i th} - 1)
+            kt[4 * m + 2] = (double)sinl(th);            // Im(e^{i th} - 1)
+            kt[4 * m + 3] = 0.0;
+        }
+        if (R.fp64) {
+            HIPCHK(hipMemcpyAsync((void *)R.D.tw[a], tw.data(), tw.size() * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync((void *)R.D.ktab[a], kt.data(), kt.size() * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+        } else {
+            std::vector<float> twf(tw.begin(), tw.end()), ktf(kt.begin(), kt.end());
+            HIPCHK(hipMemcpyAsync((void *)R.D.tw[a], twf.data(), twf.size() * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync((void *)R.D.ktab[a], ktf.data(), ktf.size() * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+    }
+    return SELM_OK;
+}
+
+static int host_set_beads(Replica &R, int M, const double *x, const double *v, const double *m,
+                          std::string &err) {
+    if (M < 0 || (M > 0 && (!x || !m))) {
+        err = "selm_set_beads: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    for (int i = 0; i < M; i++)
+        if (!(m[i] > 0)) {
+            err = "selm_set_beads: bead mass must be > 0";
+            return SELM_ERR_ARG;
+        }
+    R.M = M;
+    R.hx.assign(x, x + 3 * (size_t)M);
+    if (v) R.hv.assign(v, v + 3 * (size_t)M);
+    else R.hv.assign(3 * (size_t)M, 0.0);
+    R.hm.assign(m, m + M);
+    R.hfext.assign(3 * (size_t)M, 0.0);
+    R.adj_ptr.assign((size_t)M + 1, 0);
+    R.adj_i1.clear();
+    R.adj_i2.clear();
+    R.adj_K.clear();
+    R.adj_r0.clear();
+    R.nb = 0;
+    return SELM_OK;
+}
+
+static int host_set_bonds(Replica &R, int nb, const int *ij, const double *K, const double *r0,
+                          std::string &err) {
+    if (nb < 0 || (nb > 0 && (!ij || !K || !r0))) {
+        err = "selm_set_bonds: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    std::vector<std::vector<int>> per(R.M);
+    for (int b = 0; b < nb; b++) {
+        int i1 = ij[2 * b], i2 = ij[2 * b + 1];
+        if (i1 < 0 || i2 < 0 || i1 >= R.M || i2 >= R.M || i1 == i2) {
+            err = "selm_set_bonds: bond " + std::to_string(b) + " has invalid bead indices";
+            return SELM_ERR_ARG;
+        }
+        per[i1].push_back(b);
+        per[i2].push_back(b);
+    }
+    R.adj_ptr.assign((size_t)R.M + 1, 0);
+    R.adj_i1.clear();
+    R.adj_i2.clear();
+    R.adj_K.clear();
+    R.adj_r0.clear();
+    for (int i = 0; i < R.M; i++) {
+        R.adj_ptr[i] = (int)R.adj_i1.size();
+        for (int b : per[i]) {  // ascending bond index = LAMMPS bond-list order
+            R.adj_i1.push_back(ij[2 * b]);
+            R.adj_i2.push_back(ij[2 * b + 1]);
+            R.adj_K.push_back(K[b]);
+            R.adj_r0.push_back(r0[b]);
+        }
+    }
+    R.adj_ptr[R.M] = (int)R.adj_i1.size();
+    R.nb = nb;
+    return SELM_OK;
+}
+
+static int host_set_external(Replica &R, int n, const int *idx, const double *F, std::string &err) {
+    if (n < 0 || (n > 0 && (!idx || !F))) {
+        err = "selm_set_external_force: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    std::fill(R.hfext.begin(), R.hfext.end(), 0.0);
+    for (int a = 0; a < n; a++) {
+        if (idx[a] < 0 || idx[a] >= R.M) {
+            err = "selm_set_external_force: bead index out of range";
+            return SELM_ERR_ARG;
+        }
+        for (int d = 0; d < 3; d++) R.hfext[3 * (size_t)idx[a] + d] += F[3 * a + d];
+    }
+    return SELM_OK;
+}
+
+// Selm::initial for one replica whose memory is carved: LAMMPS-style wrap of
+// x into [lo, hi) per dimension (Domain::pbc during setup(1)), uploads, u = 0.
+static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
+    std::vector<double> xw = R.hx;
+    for (int i = 0; i < R.M; i++)
+        for (int d = 0; d < 3; d++) {
+            double lo = R.P.lo[d], hi = R.P.hi[d], L = hi - lo;
+            double &xi = xw[3 * (size_t)i + d];
+            if (xi < lo) xi += L;
+            if (xi >= hi) xi -= L;
+            xi = std::max(xi, lo);
+        }
+    const size_t b3 = 3 * (size_t)R.M * 8;
+    if (R.M > 0) {
+        HIPCHK(hipMemcpyAsync(R.D.x, xw.data(), b3, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.v, R.hv.data(), b3, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.m, R.hm.data(), (size_t)R.M * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.fext, R.hfext.data(), b3, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(R.D.ftot, 0, b3, st));
+        HIPCHK(hipMemsetAsync(R.D.fc, 0, b3, st));
+    }
+    HIPCHK(hipMemcpyAsync(R.D.adj_ptr, R.adj_ptr.data(), R.adj_ptr.size() * 4, hipMemcpyHostToDevice, st));
+    if (!R.adj_i1.empty()) {
+        size_t na = R.adj_i1.size();
+        HIPCHK(hipMemcpyAsync(R.D.adj_i1, R.adj_i1.data(), na * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.adj_i2, R.adj_i2.data(), na * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.adj_K, R.adj_K.data(), na * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.adj_r0, R.adj_r0.data(), na * 8, hipMemcpyHostToDevice, st));
+    }
+    const size_t s = R.fp64 ? 8 : 4;
+    HIPCHK(hipMemsetAsync(R.D.u, 0, 3 * (size_t)R.G.N * s, st));
+    HIPCHK(hipMemsetAsync(R.D.fsp, 0, 3 * (size_t)R.G.N * s, st));
+    HIPCHK(hipMemsetAsync(R.D.tcount, 0, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.tcursor, 0, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.ts, 0, 8, st));
+    HIPCHK(hipMemsetAsync(R.D.diag, 0, 24, st));
+    HIPCHK(hipStreamSynchronize(st));
+    R.inited = true;
+    return SELM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// A batch of replicas stepped together
+// ---------------------------------------------------------------------------
+struct Batch {
+    bool fp64 = true;
+    bool use_graph = true;
+    hipStream_t stream = nullptr;
+    std::vector<Replica *> reps;
+    RepDev *d_reps = nullptr;
+    int cap_reps = 0;
+    long long *d_tabs = nullptr;
+    int cap_tabs = 0;
+    StepLaunch L{};
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    int rebuild(std::string &err) {
+        drop_graph();
+        const int nrep = (int)reps.size();
+        if (nrep > cap_reps) {
+            if (d_reps) (void)hipFree(d_reps);
+            d_reps = nullptr;
+            int cap = std::max(nrep, 2 * cap_reps);
+            HIPCHK(hipMalloc(&d_reps, sizeof(RepDev) * cap));
+            cap_reps = cap;
+        }
+        if (6 * (nrep + 1) > cap_tabs) {
+            if (d_tabs) (void)hipFree(d_tabs);
+            d_tabs = nullptr;
+            int cap = std::max(6 * (nrep + 1), 2 * cap_tabs);
+            HIPCHK(hipMalloc(&d_tabs, sizeof(long long) * cap));
+            cap_tabs = cap;
+        }
+        std::vector<RepDev> h(nrep);
+        std::vector<long long> tabs(6 * (nrep + 1), 0);
+        long long *sb = tabs.data(), *stl = sb + (nrep + 1), *spt = stl + (nrep + 1);
+        long long *sfx = spt + (nrep + 1), *sfy = sfx + (nrep + 1), *sfz = sfy + (nrep + 1);
+        const size_t csz = fp64 ? 16 : 8;
+        int twmax = 0;
+        for (int r = 0; r < nrep; r++)
+            for (int a = 0; a < 3; a++) twmax = std::max(twmax, reps[r]->G.n[a]);
+        twmax = (twmax + 1) & ~1;
+        size_t lfx = 0, lfy = 0, lfz = 0;
+        for (int r = 0; r < nrep; r++) {
+            const Replica &R = *reps[r];
+            const Geometry &G = R.G;
+            h[r] = R.D;
+            h[r].M = R.M;
+            sb[r + 1] = sb[r] + R.M;
+            stl[r + 1] = stl[r] + G.ntiles;
+            spt[r + 1] = spt[r] + G.N;
+            long long rows = 3LL * G.n[1] * G.n[2];
+            sfx[r + 1] = sfx[r] + (rows + G.bx - 1) / G.bx;
+            sfy[r + 1] = sfy[r] + 3LL * G.n[2] * ((G.xc + G.by - 1) / G.by);
+            sfz[r + 1] = sfz[r] + ((long long)G.n[1] * G.xc + G.bz - 1) / G.bz;
+            lfx = std::max(lfx, lds_x(G, twmax, csz));
+            lfy = std::max(lfy, lds_y(G, twmax, csz));
+            lfz = std::max(lfz, lds_z(G, twmax, csz));
+        }
+        if (lfx > kLdsMax || lfy > kLdsMax || lfz > kLdsMax) {
+            err = "batch exceeds the LDS budget";
+            return SELM_ERR_UNSUPPORTED;
+        }
+        if (nrep > 0) {
+            HIPCHK(hipMemcpyAsync(d_reps, h.data(), sizeof(RepDev) * nrep, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(d_tabs, tabs.data(), sizeof(long long) * tabs.size(), hipMemcpyHostToDevice,
+                                  stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        L.reps = d_reps;
+        L.nrep = nrep;
+        L.fp64 = fp64;
+        L.st_bead = d_tabs;
+        L.st_tile = d_tabs + (nrep + 1);
+        L.st_pts = d_tabs + 2 * (nrep + 1);
+        L.st_fx = d_tabs + 3 * (nrep + 1);
+        L.st_fy = d_tabs + 4 * (nrep + 1);
+        L.st_fz = d_tabs + 5 * (nrep + 1);
+        L.tot_bead = sb[nrep];
+        L.tot_tile = stl[nrep];
+        L.tot_pts = spt[nrep];
+        L.tot_fx = sfx[nrep];
+        L.tot_fy = sfy[nrep];
+        L.tot_fz = sfz[nrep];
+        L.lds_fx = lfx;
+        L.lds_fy = lfy;
+        L.lds_fz = lfz;
+        L.twmax = twmax;
+        dirty = false;
+        return SELM_OK;
+    }
+
+    int ensure(std::string &err) {
+        if (dirty) return rebuild(err);
+        return SELM_OK;
+    }
+
+    int step(int nsteps, hipStream_t st, std::string &err) {
+        int rc = ensure(err);
+        if (rc) return rc;
+        if (reps.empty() || nsteps <= 0) return SELM_OK;
+        if (!st) st = stream;
+        if (use_graph) {
+            if (!exec) {
+                HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                launch_step(L, stream);
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamEndCapture(stream, &g));
+                graph = g;
+                HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            }
+            for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(exec, st));
+        } else {
+            for (int s = 0; s < nsteps; s++) launch_step(L, st);
+            HIPCHK(hipGetLastError());
+        }
+        return SELM_OK;
+    }
+
+    void release() {
+        drop_graph();
+        if (d_reps) (void)hipFree(d_reps);
+        if (d_tabs) (void)hipFree(d_tabs);
+        d_reps = nullptr;
+        d_tabs = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+static int download_beads(Replica &R, hipStream_t st, double *x, double *v, double *fc, double *ft,
+                          std::string &err) {
+    const size_t b3 = 3 * (size_t)R.M * 8;
+    HIPCHK(hipStreamSynchronize(st));
+    if (R.M == 0) return SELM_OK;
+    if (x) HIPCHK(hipMemcpy(x, R.D.x, b3, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(hipMemcpy(v, R.D.v, b3, hipMemcpyDeviceToHost));
+    if (fc) HIPCHK(hipMemcpy(fc, R.D.fc, b3, hipMemcpyDeviceToHost));
+    if (ft) HIPCHK(hipMemcpy(ft, R.D.ftot, b3, hipMemcpyDeviceToHost));
+    return SELM_OK;
+}
+
+static int download_field(Replica &R, hipStream_t st, int comp, double *out, std::string &err) {
+    if (comp < 0 || comp > 2 || !out) {
+        err = "get_field: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    const size_t N = (size_t)R.G.N;
+    if (R.fp64) {
+        HIPCHK(hipMemcpy(out, (double *)R.D.u + comp * N, N * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(N);
+        HIPCHK(hipMemcpy(tmp.data(), (float *)R.D.u + comp * N, N * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) out[i] = tmp[i];
+    }
+    return SELM_OK;
+}
+
+}  // namespace selm
+
+using namespace selm;
+
+// ===========================================================================
+// selm_ctx: one system
+// ===========================================================================
+struct selm_ctx {
+    Batch batch;
+    Replica rep;
+    void *grid_mem = nullptr;
+    void *bead_mem = nullptr;
+    std::string err;
+    hipStream_t last = nullptr;
+};
+
+#define CTX_CHECK(c)                  \
+    do {                              \
+        if (!(c)) return SELM_ERR_ARG; \
+    } while (0)
+
+extern "C" {
+
+int selm_params_default(selm_params *p) {
+    if (!p) return SELM_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->dims[0] = p->dims[1] = p->dims[2] = 32;
+    for (int d = 0; d < 3; d++) {
+        p->lo[d] = 0.0;
+        p->hi[d] = 32.0;
+    }
+    p->rho = p->mu = p->kB = p->T = 1.0;
+    p->dt = 0.01;
+    p->seed = 123456;
+    p->stochastic = 0;
+    p->precision = SELM_PREC_FP64;
+    p->f4_quirk = 1;
+    return SELM_OK;
+}
+
+int selm_create(const selm_params *p, selm_ctx **out) {
+    if (!p || !out) return SELM_ERR_ARG;
+    *out = nullptr;
+    if (p->precision != SELM_PREC_FP64 && p->precision != SELM_PREC_FP32_BULK) return SELM_ERR_ARG;
+    selm_ctx *c = new (std::nothrow) selm_ctx();
+    if (!c) return SELM_ERR_NOMEM;
+    std::string &err = c->err;
+    const bool fp64 = p->precision == SELM_PREC_FP64;
+    int rc = replica_setup(c->rep, *p, fp64, err);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    c->batch.fp64 = fp64;
+    hipError_t e = hipStreamCreateWithFlags(&c->batch.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return SELM_ERR_HIP;
+    }
+    size_t gb = grid_bytes(c->rep.G, fp64);
+    e = hipMalloc(&c->grid_mem, gb);
+    if (e != hipSuccess) {
+        c->batch.release();
+        delete c;
+        return SELM_ERR_NOMEM;
+    }
+    c->rep.grid_sz = gb;
+    carve_grid(c->rep, (char *)c->grid_mem);
+    rc = upload_tables(c->rep, c->batch.stream, err);
+    if (rc) {
+        (void)hipFree(c->grid_mem);
+        c->batch.release();
+        delete c;
+        return rc;
+    }
+    // zero-bead system until selm_set_beads
+    host_set_beads(c->rep, 0, nullptr, nullptr, nullptr, err);
+    c->batch.reps.push_back(&c->rep);
+    c->last = c->batch.stream;
+    *out = c;
+    return SELM_OK;
+}
+
+int selm_destroy(selm_ctx *c) {
+    if (!c) return SELM_ERR_ARG;
+    if (c->batch.stream) (void)hipStreamSynchronize(c->batch.stream);
+    if (c->last && c->last != c->batch.stream) (void)hipStreamSynchronize(c->last);
+    c->batch.release();
+    if (c->grid_mem) (void)hipFree(c->grid_mem);
+    if (c->bead_mem) (void)hipFree(c->bead_mem);
+    delete c;
+    return SELM_OK;
+}
+
+const char *selm_last_error(const selm_ctx *c) { return c ? c->err.c_str() : "null ctx"; }
+
+static int ctx_alloc_beads(selm_ctx *c) {
+    std::string &err = c->err;
+    if (c->bead_mem) {
+        HIPCHK(hipStreamSynchronize(c->batch.stream));
+        (void)hipFree(c->bead_mem);
+        c->bead_mem = nullptr;
+    }
+    size_t bb = bead_bytes(c->rep.M, (int)c->rep.adj_i1.size());
+    c->rep.bead_sz = bb;
+    if (hipMalloc(&c->bead_mem, std::max<size_t>(bb, 256)) != hipSuccess) {
+        err = "hipMalloc of bead buffers failed";
+        return SELM_ERR_NOMEM;
+    }
+    carve_beads(c->rep, (char *)c->bead_mem);
+    c->batch.dirty = true;
+    c->rep.inited = false;
+    return SELM_OK;
+}
+
+int selm_set_beads(selm_ctx *c, int M, const double *x, const double *v, const double *m) {
+    CTX_CHECK(c);
+    int rc = host_set_beads(c->rep, M, x, v, m, c->err);
+    if (rc) return rc;
+    return ctx_alloc_beads(c);
+}
+
+int selm_set_bonds(selm_ctx *c, int nb, const int *ij, const double *K, const double *r0) {
+    CTX_CHECK(c);
+    int rc = host_set_bonds(c->rep, nb, ij, K, r0, c->err);
+    if (rc) return rc;
+    return ctx_alloc_beads(c);
+}
+
+int selm_set_external_force(selm_ctx *c, int n, const int *idx, const double *F) {
+    CTX_CHECK(c);
+    int rc = host_set_external(c->rep, n, idx, F, c->err);
+    if (rc) return rc;
+    c->rep.inited = false;
+    return SELM_OK;
+}
+
+int selm_set_conservative_force(selm_ctx *c, const double *f) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    if (!c->rep.inited) {
+        err = "selm_set_conservative_force before selm_init";
+        return SELM_ERR_STATE;
+    }
+    if (c->rep.M == 0) return SELM_OK;
+    if (!f) return SELM_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->last));
+    HIPCHK(hipMemcpy(c->rep.D.fc, f, 3 * (size_t)c->rep.M * 8, hipMemcpyHostToDevice));
+    return SELM_OK;
+}
+
+int selm_init(selm_ctx *c) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    if (!c->bead_mem) {
+        int rc = ctx_alloc_beads(c);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->last));
+    int rc = replica_upload_init(c->rep, c->batch.stream, err);
+    if (rc) return rc;
+    rc = c->batch.ensure(err);
+    if (rc) return rc;
+    launch_bond_forces(c->batch.L, 0, c->batch.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->batch.stream));
+    c->last = c->batch.stream;
+    return SELM_OK;
+}
+
+int selm_step(selm_ctx *c, int nsteps, void *stream) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    if (!c->rep.inited) {
+        err = "selm_step before selm_init";
+        return SELM_ERR_STATE;
+    }
+    if (nsteps < 0) return SELM_ERR_ARG;
+    hipStream_t st = stream ? (hipStream_t)stream : c->batch.stream;
+    if (st != c->last) HIPCHK(hipStreamSynchronize(c->last));
+    c->last = st;
+    return c->batch.step(nsteps, st, err);
+}
+
+int selm_sync(selm_ctx *c) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    HIPCHK(hipStreamSynchronize(c->last));
+    return SELM_OK;
+}
+
+int selm_get_beads(selm_ctx *c, double *x, double *v, double *fc, double *ft) {
+    CTX_CHECK(c);
+    if (!c->bead_mem) return SELM_OK;
+    return download_beads(c->rep, c->last, x, v, fc, ft, c->err);
+}
+
+int selm_get_field(selm_ctx *c, int comp, double *u) {
+    CTX_CHECK(c);
+    return download_field(c->rep, c->last, comp, u, c->err);
+}
+
+int selm_set_field(selm_ctx *c, int comp, const double *u) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    if (comp < 0 || comp > 2 || !u) return SELM_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->last));
+    const size_t N = (size_t)c->rep.G.N;
+    if (c->rep.fp64) {
+        HIPCHK(hipMemcpy((double *)c->rep.D.u + comp * N, u, N * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(u, u + N);
+        HIPCHK(hipMemcpy((float *)c->rep.D.u + comp * N, tmp.data(), N * 4, hipMemcpyHostToDevice));
+    }
+    return SELM_OK;
+}
+
+long long selm_timestep(selm_ctx *c) {
+    if (!c || !c->rep.D.ts) return -1;
+    long long ts = -1;
+    if (hipStreamSynchronize(c->last) != hipSuccess) return -1;
+    if (hipMemcpy(&ts, c->rep.D.ts, 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return ts;
+}
+
+int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes) {
+    if (!p || !bytes || M < 0 || nb < 0) return SELM_ERR_ARG;
+    Geometry G;
+    std::string err;
+    const bool fp64 = p->precision == SELM_PREC_FP64;
+    int rc = make_geometry(*p, fp64, G, err);
+    if (rc) return rc;
+    *bytes = grid_bytes(G, fp64) + bead_bytes(M, 2 * nb);
+    return SELM_OK;
+}
+
+int selm_stage_times(selm_ctx *c, int nsteps, double *ms, int nstages) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    if (!ms || nstages < kNumStages || nsteps < 0) return SELM_ERR_ARG;
+    if (!c->rep.inited) {
+        err = "selm_stage_times before selm_init";
+        return SELM_ERR_STATE;
+    }
+    int rc = c->batch.ensure(err);
+    if (rc) return rc;
+    hipStream_t st = c->batch.stream;
+    HIPCHK(hipStreamSynchronize(c->last));
+    c->last = st;
+    std::vector<hipEvent_t> ev(kNumStages + 1);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (int s = 0; s < kNumStages; s++) ms[s] = 0.0;
+    for (int it = 0; it < nsteps; it++) {
+        HIPCHK(hipEventRecord(ev[0], st));
+        for (int s = 0; s < kNumStages; s++) {
+            launch_stage(c->batch.L, s, st);
+            HIPCHK(hipEventRecord(ev[s + 1], st));
+        }
+        HIPCHK(hipEventSynchronize(ev[kNumStages]));
+        for (int s = 0; s < kNumStages; s++) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[s], ev[s + 1]));
+            ms[s] += t;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    HIPCHK(hipGetLastError());
+    return SELM_OK;
+}
+
+int selm_mean_mode(selm_ctx *c, double out[3]) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    HIPCHK(hipStreamSynchronize(c->last));
+    HIPCHK(hipMemcpy(out, c->rep.D.diag, 24, hipMemcpyDeviceToHost));
+    return SELM_OK;
+}
+
+void *selm_stream(selm_ctx *c) { return c ? (void *)c->batch.stream : nullptr; }
+
+}  // extern "C"
+
+// ===========================================================================
+// selm_pool: many systems in one device arena, stepped together
+// ===========================================================================
+namespace selm {
+struct ArenaAlloc {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+    void init(size_t s) {
+        size = s;
+        free_.clear();
+        free_[0] = s;
+    }
+    bool alloc(size_t bytes, size_t &off) {
+        bytes = al(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                off = it->first;
+                size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = al(bytes);
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const {
+        size_t f = 0;
+        for (auto &kv : free_) f += kv.second;
+        return size - f;
+    }
+};
+}  // namespace selm
+
+struct selm_pool {
+    Batch batch;
+    char *arena = nullptr;
+    ArenaAlloc alloc;
+    std::vector<Replica *> slots;
+    std::vector<size_t> slot_off, slot_bytes;
+    std::string err;
+    int precision = 0;
+};
+
+extern "C" {
+
+int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
+    if (!pp || !out || pp->max_replicas <= 0 || pp->hbm_budget < 4096) return SELM_ERR_ARG;
+    if (pp->precision != SELM_PREC_FP64 && pp->precision != SELM_PREC_FP32_BULK) return SELM_ERR_ARG;
+    *out = nullptr;
+    selm_pool *P = new (std::nothrow) selm_pool();
+    if (!P) return SELM_ERR_NOMEM;
+    P->precision = pp->precision;
+    P->batch.fp64 = pp->precision == SELM_PREC_FP64;
+    P->batch.use_graph = pp->use_graph != 0;
+    if (hipStreamCreateWithFlags(&P->batch.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete P;
+        return SELM_ERR_HIP;
+    }
+    size_t bytes = al(pp->hbm_budget);
+    if (hipMalloc((void **)&P->arena, bytes) != hipSuccess) {
+        P->batch.release();
+        delete P;
+        return SELM_ERR_NOMEM;
+    }
+    P->alloc.init(bytes);
+    P->slots.assign(pp->max_replicas, nullptr);
+    P->slot_off.assign(pp->max_replicas, 0);
+    P->slot_bytes.assign(pp->max_replicas, 0);
+    *out = P;
+    return SELM_OK;
+}
+
+int selm_pool_destroy(selm_pool *P) {
+    if (!P) return SELM_ERR_ARG;
+    (void)hipStreamSynchronize(P->batch.stream);
+    for (Replica *r : P->slots) delete r;
+    P->batch.release();
+    if (P->arena) (void)hipFree(P->arena);
+    delete P;
+    return SELM_OK;
+}
+
+const char *selm_pool_last_error(const selm_pool *P) { return P ? P->err.c_str() : "null pool"; }
+
+static void pool_rebuild_list(selm_pool *P) {
+    P->batch.reps.clear();
+    for (Replica *r : P->slots)
+        if (r) P->batch.reps.push_back(r);
+    P->batch.dirty = true;
+}
+
+int selm_pool_admit(selm_pool *P, const selm_params *p, int M, const double *x, const double *v,
+                    const double *m, int nb, const int *ij, const double *K, const double *r0, int next,
+                    const int *idx, const double *F, int *slot) {
+    if (!P || !p || !slot) return SELM_ERR_ARG;
+    std::string &err = P->err;
+    int s = -1;
+    for (size_t q = 0; q < P->slots.size(); q++)
+        if (!P->slots[q]) {
+            s = (int)q;
+            break;
+        }
+    if (s < 0) {
+        err = "no free slot";
+        return SELM_ERR_POOL_FULL;
+    }
+    Replica *R = new (std::nothrow) Replica();
+    if (!R) return SELM_ERR_NOMEM;
+    selm_params q = *p;
+    q.precision = P->precision;
+    const bool fp64 = P->batch.fp64;
+    int rc = replica_setup(*R, q, fp64, err);
+    if (!rc) rc = host_set_beads(*R, M, x, v, m, err);
+    if (!rc) rc = host_set_bonds(*R, nb, ij, K, r0, err);
+    if (!rc) rc = host_set_external(*R, next, idx, F, err);
+    if (rc) {
+        delete R;
+        return rc;
+    }
+    size_t gb = grid_bytes(R->G, fp64), bb = bead_bytes(R->M, (int)R->adj_i1.size());
+    size_t off = 0;
+    if (!P->alloc.alloc(gb + bb, off)) {
+        err = "not enough free arena space for footprint " + std::to_string(gb + bb);
+        delete R;
+        return SELM_ERR_POOL_FULL;
+    }
+    HIPCHK(hipStreamSynchronize(P->batch.stream));
+    carve_grid(*R, P->arena + off);
+    carve_beads(*R, P->arena + off + al(gb));
+    R->grid_sz = gb;
+    R->bead_sz = bb;
+    rc = upload_tables(*R, P->batch.stream, err);
+    if (!rc) rc = replica_upload_init(*R, P->batch.stream, err);
+    if (rc) {
+        P->alloc.release(off, gb + bb);
+        delete R;
+        return rc;
+    }
+    P->slots[s] = R;
+    P->slot_off[s] = off;
+    P->slot_bytes[s] = gb + bb;
+    pool_rebuild_list(P);
+    rc = P->batch.ensure(err);
+    if (rc) return rc;
+    // forces at x^0 for all systems (recomputing the others' is idempotent)
+    launch_bond_forces(P->batch.L, 0, P->batch.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(P->batch.stream));
+    *slot = s;
+    return SELM_OK;
+}
+
+int selm_pool_step(selm_pool *P, int nsteps) {
+    if (!P) return SELM_ERR_ARG;
+    return P->batch.step(nsteps, P->batch.stream, P->err);
+}
+
+int selm_pool_sync(selm_pool *P) {
+    if (!P) return SELM_ERR_ARG;
+    std::string &err = P->err;
+    HIPCHK(hipStreamSynchronize(P->batch.stream));
+    return SELM_OK;
+}
+
+static Replica *pool_slot(selm_pool *P, int slot) {
+    if (!P || slot < 0 || slot >= (int)P->slots.size()) return nullptr;
+    return P->slots[slot];
+}
+
+int selm_pool_get_beads(selm_pool *P, int slot, double *x, double *v, double *fc, double *ft) {
+    Replica *R = pool_slot(P, slot);
+    if (!R) return SELM_ERR_ARG;
+    return download_beads(*R, P->batch.stream, x, v, fc, ft, P->err);
+}
+
+int selm_pool_get_field(selm_pool *P, int slot, int comp, double *u) {
+    Replica *R = pool_slot(P, slot);
+    if (!R) return SELM_ERR_ARG;
+    return download_field(*R, P->batch.stream, comp, u, P->err);
+}
+
+long long selm_pool_timestep(selm_pool *P, int slot) {
+    Replica *R = pool_slot(P, slot);
+    if (!R) return -1;
+    long long ts = -1;
+    if (hipStreamSynchronize(P->batch.stream) != hipSuccess) return -1;
+    if (hipMemcpy(&ts, R->D.ts, 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return ts;
+}
+
+int selm_pool_retire(selm_pool *P, int slot) {
+    Replica *R = pool_slot(P, slot);
+    if (!R) return SELM_ERR_ARG;
+    std::string &err = P->err;
+    HIPCHK(hipStreamSynchronize(P->batch.stream));
+    P->alloc.release(P->slot_off[slot], P->slot_bytes[slot]);
+    delete R;
+    P->slots[slot] = nullptr;
+    pool_rebuild_list(P);
+    return SELM_OK;
+}
+
+int selm_pool_stats(selm_pool *P, size_t *used, size_t *budget, int *active) {
+    if (!P) return SELM_ERR_ARG;
+    if (used) *used = P->alloc.used();
+    if (budget) *budget = P->alloc.size;
+    if (active) *active = (int)P->batch.reps.size();
+    return SELM_OK;
+}
+
+void *selm_pool_stream(selm_pool *P) { return P ? (void *)P->batch.stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// selm_kernels.hpp -- launch interface between the host engine and kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "selm_desc.hpp"
+
+namespace selm {
+
+enum Stage {
+    kStageBead = 0,
+    kStageBin,
+    kStageSpread,
+    kStageUstar,
+    kStageFftX,
+    kStageFftY,
+    kStageFftZ,
+    kStageFftYi,
+    kStageFftXi,
+    kStageBonds,
+    kNumStages
+};
+
+// Everything one batched step launch needs (device pointers + totals).
+struct StepLaunch {
+    const RepDev *reps;   // device array [nrep]
+    int nrep;
+    bool fp64;
+    const long long *st_bead, *st_tile, *st_pts, *st_fx, *st_fy, *st_fz;  // device [nrep+1]
+    long long tot_bead, tot_tile, tot_pts, tot_fx, tot_fy, tot_fz;
+    size_t lds_fx, lds_fy, lds_fz;   // dynamic LDS bytes per pass
+    int twmax;                       // LDS twiddle slots (elements) preceding the data
+};
+
+void launch_step(const StepLaunch &L, hipStream_t st);
+void launch_stage(const StepLaunch &L, int stage, hipStream_t st);
+void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st);
+void launch_fill_f64(double *p, long long n, double v, hipStream_t st);
+void launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+
+}  // namespace selm
