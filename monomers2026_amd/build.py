@@ -22,7 +22,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-Wno-unused-command-line-argument"]
 
-LIB_SOURCES = ["selm_kernels.hip", "selm_engine.cpp"]
+LIB_SOURCES = ["selm_kernels.hip", "selm_fft_kernels.hip", "selm_engine.cpp"]
 DRIVER_SOURCES = ["host/selm_main.cpp", "host/SELM.cpp", "host/SELM_Eulerian_Period_HIP.cpp",
                   "host/SELM_Lagrangian_Chain_HIP.cpp", "host/lammps_subset.cpp", "host/dcd_writer.cpp"]
 

@@ -31,7 +31,9 @@ struct RepDev {
     // ---- beads
     int M;
     double *x, *v, *fc, *ftot, *m, *fext, *tf, *sg;
-    int *key, *tcount, *tstart, *tcursor, *permu, *perm;
+    void *brec;                // SpreadRec<T>[M]: per-bead stencil record for the spread
+    int *tcount, *tstart, *tcursor;   // per tile: bucket counts, starts (ntiles+1), fill cursors
+    int *permu, *etile, *perm;        // bucket entries [27*M]: unordered, their tile, ordered
     int *adj_ptr, *adj_i1, *adj_i2;   // CSR over beads: bonds touching each bead, bond order
     double *adj_K, *adj_r0;
     // ---- scalars
@@ -47,6 +49,21 @@ struct RepDev {
     long long *ts;             // completed steps (device counter)
     double *diag;              // [3] k=0 mode of w (momentum diagnostic), FP64
 };
+
+// Per-bead record written by the bead kernel and read by the spread: for each
+// staggered component d and axis a, the first stencil index (wrapped) and the
+// four 1-D Peskin factors; and -t_f (the impulse the fluid receives).
+template <typename T>
+struct SpreadRec {
+    int p[3][4];
+    T w[3][3][4];
+    T f[4];
+};
+
+// A bead is binned into every tile its stencils touch: on each axis the union
+// of the three staggered stencils spans cells floor(s-1/2)-1 .. floor(s)+2,
+// which meets at most 3 tiles (2 unless a ragged tile is narrower than 4).
+constexpr int kMaxTilesPerBead = 27;
 
 // Work table for one stage: item g of the launch belongs to replica r with
 // start[r] <= g < start[r+1].

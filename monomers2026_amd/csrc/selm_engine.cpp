@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <new>
@@ -119,11 +120,12 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     return b;
 }
 
-static size_t bead_bytes(int M, int nadj) {
+static size_t bead_bytes(int M, int nadj, bool fp64) {
     size_t b = 0;
     b += 7 * al(3 * (size_t)M * 8);
     b += al((size_t)M * 8);
-    b += 3 * al((size_t)M * 4);
+    b += al((size_t)M * spread_rec_bytes(fp64));
+    b += 3 * al((size_t)kMaxTilesPerBead * M * 4);
     b += al(((size_t)M + 1) * 4);
     b += 2 * al((size_t)nadj * 4) + 2 * al((size_t)nadj * 8);
     return b;
@@ -189,9 +191,10 @@ static void carve_beads(Replica &R, char *base) {
     R.D.tf = (double *)take(3 * (size_t)M * 8);
     R.D.sg = (double *)take(3 * (size_t)M * 8);
     R.D.m = (double *)take((size_t)M * 8);
-    R.D.key = (int *)take((size_t)M * 4);
-    R.D.permu = (int *)take((size_t)M * 4);
-    R.D.perm = (int *)take((size_t)M * 4);
+    R.D.brec = take((size_t)M * spread_rec_bytes(R.fp64));
+    R.D.permu = (int *)take((size_t)kMaxTilesPerBead * M * 4);
+    R.D.etile = (int *)take((size_t)kMaxTilesPerBead * M * 4);
+    R.D.perm = (int *)take((size_t)kMaxTilesPerBead * M * 4);
     R.D.adj_ptr = (int *)take(((size_t)M + 1) * 4);
     R.D.adj_i1 = (int *)take(nadj * 4);
     R.D.adj_i2 = (int *)take(nadj * 4);
@@ -404,6 +407,7 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
 struct Batch {
     bool fp64 = true;
     bool use_graph = true;
+    bool force_generic_fft = false;
     hipStream_t stream = nullptr;
     std::vector<Replica *> reps;
     RepDev *d_reps = nullptr;
@@ -441,29 +445,48 @@ struct Batch {
         }
         std::vector<RepDev> h(nrep);
         std::vector<long long> tabs(6 * (nrep + 1), 0);
-        long long *sb = tabs.data(), *stl = sb + (nrep + 1), *spt = stl + (nrep + 1);
-        long long *sfx = spt + (nrep + 1), *sfy = sfx + (nrep + 1), *sfz = sfy + (nrep + 1);
+        long long *sb = tabs.data(), *stl = sb + (nrep + 1), *sen = stl + (nrep + 1);
+        long long *sfx = sen + (nrep + 1), *sfy = sfx + (nrep + 1), *sfz = sfy + (nrep + 1);
         const size_t csz = fp64 ? 16 : 8;
         int twmax = 0;
         for (int r = 0; r < nrep; r++)
             for (int a = 0; a < 3; a++) twmax = std::max(twmax, reps[r]->G.n[a]);
         twmax = (twmax + 1) & ~1;
         size_t lfx = 0, lfy = 0, lfz = 0;
+        // compile-time-length FFT kernels when every replica's edges have one
+        bool fast = nrep > 0 && !force_generic_fft;
+        for (int r = 0; r < nrep; r++) {
+            size_t a, b, c;
+            fft_fast_lds(fp64, reps[r]->G.n, twmax, &a, &b, &c);
+            fast = fast && fft_fast_supported(reps[r]->G.n) && a <= kLdsMax && b <= kLdsMax && c <= kLdsMax;
+        }
         for (int r = 0; r < nrep; r++) {
             const Replica &R = *reps[r];
-            const Geometry &G = R.G;
+            Geometry G = R.G;
+            if (fast) fft_fast_blocks(fp64, &G.bx, &G.by, &G.bz);
             h[r] = R.D;
             h[r].M = R.M;
+            h[r].bx = G.bx;
+            h[r].by = G.by;
+            h[r].bz = G.bz;
             sb[r + 1] = sb[r] + R.M;
             stl[r + 1] = stl[r] + G.ntiles;
-            spt[r + 1] = spt[r] + G.N;
+            sen[r + 1] = sen[r] + (long long)kMaxTilesPerBead * R.M;
             long long rows = 3LL * G.n[1] * G.n[2];
             sfx[r + 1] = sfx[r] + (rows + G.bx - 1) / G.bx;
             sfy[r + 1] = sfy[r] + 3LL * G.n[2] * ((G.xc + G.by - 1) / G.by);
             sfz[r + 1] = sfz[r] + ((long long)G.n[1] * G.xc + G.bz - 1) / G.bz;
-            lfx = std::max(lfx, lds_x(G, twmax, csz));
-            lfy = std::max(lfy, lds_y(G, twmax, csz));
-            lfz = std::max(lfz, lds_z(G, twmax, csz));
+            if (fast) {
+                size_t a, b, c;
+                fft_fast_lds(fp64, G.n, twmax, &a, &b, &c);
+                lfx = std::max(lfx, a);
+                lfy = std::max(lfy, b);
+                lfz = std::max(lfz, c);
+            } else {
+                lfx = std::max(lfx, lds_x(G, twmax, csz));
+                lfy = std::max(lfy, lds_y(G, twmax, csz));
+                lfz = std::max(lfz, lds_z(G, twmax, csz));
+            }
         }
         if (lfx > kLdsMax || lfy > kLdsMax || lfz > kLdsMax) {
             err = "batch exceeds the LDS budget";
@@ -480,13 +503,13 @@ struct Batch {
         L.fp64 = fp64;
         L.st_bead = d_tabs;
         L.st_tile = d_tabs + (nrep + 1);
-        L.st_pts = d_tabs + 2 * (nrep + 1);
+        L.st_ent = d_tabs + 2 * (nrep + 1);
         L.st_fx = d_tabs + 3 * (nrep + 1);
         L.st_fy = d_tabs + 4 * (nrep + 1);
         L.st_fz = d_tabs + 5 * (nrep + 1);
         L.tot_bead = sb[nrep];
         L.tot_tile = stl[nrep];
-        L.tot_pts = spt[nrep];
+        L.tot_ent = sen[nrep];
         L.tot_fx = sfx[nrep];
         L.tot_fy = sfy[nrep];
         L.tot_fz = sfz[nrep];
@@ -494,6 +517,7 @@ struct Batch {
         L.lds_fy = lfy;
         L.lds_fz = lfz;
         L.twmax = twmax;
+        L.fast = fast;
         dirty = false;
         return SELM_OK;
     }
@@ -619,6 +643,8 @@ int selm_create(const selm_params *p, selm_ctx **out) {
         return rc;
     }
     c->batch.fp64 = fp64;
+    // SELM_FFT_GENERIC=1 selects the runtime-length FFT kernels for every size
+    c->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
     hipError_t e = hipStreamCreateWithFlags(&c->batch.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -668,7 +694,7 @@ static int ctx_alloc_beads(selm_ctx *c) {
         (void)hipFree(c->bead_mem);
         c->bead_mem = nullptr;
     }
-    size_t bb = bead_bytes(c->rep.M, (int)c->rep.adj_i1.size());
+    size_t bb = bead_bytes(c->rep.M, (int)c->rep.adj_i1.size(), c->rep.fp64);
     c->rep.bead_sz = bb;
     if (hipMalloc(&c->bead_mem, std::max<size_t>(bb, 256)) != hipSuccess) {
         err = "hipMalloc of bead buffers failed";
@@ -797,7 +823,7 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes) {
     const bool fp64 = p->precision == SELM_PREC_FP64;
     int rc = make_geometry(*p, fp64, G, err);
     if (rc) return rc;
-    *bytes = grid_bytes(G, fp64) + bead_bytes(M, 2 * nb);
+    *bytes = grid_bytes(G, fp64) + bead_bytes(M, 2 * nb, fp64);
     return SELM_OK;
 }
 
@@ -917,6 +943,7 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->precision = pp->precision;
     P->batch.fp64 = pp->precision == SELM_PREC_FP64;
     P->batch.use_graph = pp->use_graph != 0;
+    P->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
     if (hipStreamCreateWithFlags(&P->batch.stream, hipStreamNonBlocking) != hipSuccess) {
         delete P;
         return SELM_ERR_HIP;
@@ -982,7 +1009,7 @@ int selm_pool_admit(selm_pool *P, const selm_params *p, int M, const double *x, 
         delete R;
         return rc;
     }
-    size_t gb = grid_bytes(R->G, fp64), bb = bead_bytes(R->M, (int)R->adj_i1.size());
+    size_t gb = grid_bytes(R->G, fp64), bb = bead_bytes(R->M, (int)R->adj_i1.size(), fp64);
     size_t off = 0;
     if (!P->alloc.alloc(gb + bb, off)) {
         err = "not enough free arena space for footprint " + std::to_string(gb + bb);

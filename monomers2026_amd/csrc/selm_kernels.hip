@@ -4,27 +4,54 @@
 // launch advances any number of heterogeneous replicas (a single system is a
 // batch of one).  Per step, in order:
 //
-//   k_bead_update   wave per bead: 4x4x4 Peskin gather of u^n for the three
-//                   staggered components (64 lanes = 64 stencil points, wave
-//                   reduction), drag + thermal kick, x/v/f update, tile key
-//   k_bin_scan      workgroup per replica: tile counts -> bucket starts
-//   k_bin_scatter   thread per bead: bucket fill (arbitrary order)
-//   k_bin_rank      thread per bead: bucket order := ascending bead index
-//   k_spread        workgroup per tile: owner-computes segmented reduction
-//                   over the beads of the 27 neighbour buckets (fixed order,
-//                   no atomics) -> fsp
-//   k_ustar         u* = u + (mu dt L7 u + fsp) / rho, in place over fsp
-//   k_fft_x_r2c     rows along x, two real rows per complex transform
-//   k_fft_y         columns along y
-//   k_fft_z_proj    columns along z: forward DFT, thermal forcing in k-space,
-//                   discrete Helmholtz projection, k=0 removal, 1/N, inverse
-//   k_fft_y         columns along y (inverse)
-//   k_fft_x_c2r     rows along x -> u^{n+1}
-//   k_bond_forces   thread per bead: harmonic bonds + external force at
-//                   x^{n+1}; advances each replica's step counter
+//   k_bead_update     wave per bead: 4x4x4 Peskin gather of u^n for the three
+//                     staggered components (64 lanes = 64 stencil points,
+//                     wave reduction), drag + thermal kick, x/v/f update; the
+//                     bead's spread record; tile counts for binning
+//   k_bin_scan        workgroup per replica: tile counts -> bucket starts
+//   k_bin_scatter     thread per bead: bucket fill (arbitrary order)
+//   k_bin_rank        thread per bucket entry: order := ascending bead index
+//   k_spread_ustar    workgroup per tile: wave d accumulates component d of
+//                     the tile's beads in bucket order into LDS (lanes =
+//                     stencil points, distinct cells, no atomics), then
+//                     u* = u + (mu dt L7 u + f) / rho for the tile's cells
+//   k_fft_x_r2c       rows along x, two real rows per complex transform
+//   k_fft_y           columns along y
+//   k_fft_z_proj      columns along z: forward DFT, thermal forcing in k-space,
+//                     discrete Helmholtz projection, k=0 removal, 1/N, inverse
+//   k_fft_y           columns along y (inverse)
+//   k_fft_x_c2r       rows along x -> u^{n+1}
+//   k_bond_forces     thread per bead: harmonic bonds + external force at
+//                     x^{n+1}; advances each replica's step counter
 #include "selm_kernels.hpp"
 
 namespace selm {
+
+// Tiles met on one axis by the union of a bead's three staggered stencils.
+__device__ __forceinline__ int axis_tiles(double s, int n, int te, int *out) {
+    const int lo = (int)(floor(s - 0.5) - 1.0);
+    const int hi = (int)floor(s) + 2;
+    int cnt = 0;
+    for (int c = lo; c <= hi && c - lo < 6; c++) {
+        const int t = wrap_idx(c, n) / te;
+        bool dup = false;
+        for (int q = 0; q < cnt; q++) dup |= out[q] == t;
+        if (!dup) out[cnt++] = t;
+    }
+    return cnt;
+}
+
+__device__ __forceinline__ int bead_tiles(const RepDev &R, double s0, double s1, double s2, int *tiles) {
+    int tx[6], ty[6], tz[6];
+    const int cx = axis_tiles(s0, R.n[0], R.te[0], tx);
+    const int cy = axis_tiles(s1, R.n[1], R.te[1], ty);
+    const int cz = axis_tiles(s2, R.n[2], R.te[2], tz);
+    int k = 0;
+    for (int c = 0; c < cz; c++)
+        for (int b = 0; b < cy; b++)
+            for (int a = 0; a < cx; a++) tiles[k++] = (tz[c] * R.tn[1] + ty[b]) * R.tn[0] + tx[a];
+    return k;
+}
 
 // ===========================================================================
 // Beads: gather, drag, thermal kick, update.  Semantics of
@@ -43,6 +70,7 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const long long N = R.N;
     const int f4 = R.f4;
+    SpreadRec<T> *rec = (SpreadRec<T> *)R.brec + b;
 
     double s[3];
 #pragma unroll
@@ -54,29 +82,32 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         // component d is sampled at its face centre: the other two axes shift by -1/2
-        double rx = s[0] - (d != 0 ? 0.5 : 0.0);
-        double ry = s[1] - (d != 1 ? 0.5 : 0.0);
-        double rz = s[2] - (d != 2 ? 0.5 : 0.0);
-        int pi = (int)(floor(rx) - 1.0) + li;
-        int pj = (int)(floor(ry) - 1.0) + lj;
-        int pk = (int)(floor(rz) - 1.0) + lk;
-        double wz = 1.0 * peskin_phi(fabs(rz - pk), f4);
-        double wy = wz * peskin_phi(fabs(ry - pj), f4);
-        double w = wy * peskin_phi(fabs(rx - pi), f4);
-        long long idx = ((long long)wrap_idx(pk, nz) * ny + wrap_idx(pj, ny)) * nx + wrap_idx(pi, nx);
-        T val = u[d * N + idx] * (T)w;
+        const double rx = s[0] - (d != 0 ? 0.5 : 0.0);
+        const double ry = s[1] - (d != 1 ? 0.5 : 0.0);
+        const double rz = s[2] - (d != 2 ? 0.5 : 0.0);
+        const int p0 = (int)(floor(rx) - 1.0), p1 = (int)(floor(ry) - 1.0), p2 = (int)(floor(rz) - 1.0);
+        const int pi = p0 + li, pj = p1 + lj, pk = p2 + lk;
+        const double fz = peskin_phi(fabs(rz - pk), f4);
+        const double fy = peskin_phi(fabs(ry - pj), f4);
+        const double fx = peskin_phi(fabs(rx - pi), f4);
+        const double w = ((1.0 * fz) * fy) * fx;
+        const long long idx = ((long long)wrap_idx(pk, nz) * ny + wrap_idx(pj, ny)) * nx + wrap_idx(pi, nx);
+        const T val = u[d * N + idx] * (T)w;
         gam[d] = (double)wave_sum<T>(val);
+        if (lj == 0 && lk == 0) rec->w[d][0][li] = (T)fx;
+        if (li == 0 && lk == 0) rec->w[d][1][lj] = (T)fy;
+        if (li == 0 && lj == 0) rec->w[d][2][lk] = (T)fz;
+        if (lane == 0) {
+            rec->p[d][0] = wrap_idx(p0, nx);
+            rec->p[d][1] = wrap_idx(p1, ny);
+            rec->p[d][2] = wrap_idx(p2, nz);
+        }
     }
 
     if (lane == 0) {
-        // bin by the base cell of the unshifted coordinate; every stencil of
-        // this bead lies within [c-2, c+2] on each axis
-        int c0 = wrap_idx((int)floor(s[0]), nx);
-        int c1 = wrap_idx((int)floor(s[1]), ny);
-        int c2 = wrap_idx((int)floor(s[2]), nz);
-        int t = ((c2 / R.te[2]) * R.tn[1] + (c1 / R.te[1])) * R.tn[0] + (c0 / R.te[0]);
-        R.key[b] = t;
-        atomicAdd(&R.tcount[t], 1);
+        int tiles[kMaxTilesPerBead];
+        const int nt = bead_tiles(R, s[0], s[1], s[2], tiles);
+        for (int q = 0; q < nt; q++) atomicAdd(&R.tcount[tiles[q]], 1);
     }
     if (lane < 3) {
         const int d = lane;
@@ -94,7 +125,8 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
         }
         R.tf[i] = tf;
         R.sg[i] = sd;
-        double ft = tf + dt * R.fc[i];
+        rec->f[d] = (T)(-tf);
+        const double ft = tf + dt * R.fc[i];
         R.ftot[i] = ft / dt;
         double tmp = vi * dt;
         tmp += R.x[i];
@@ -106,37 +138,45 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
 }
 
 // ===========================================================================
-// Binning: counting sort of beads by owning tile, deterministic bucket order.
+// Binning: counting sort of (bead, tile) entries by tile, deterministic order.
 // ===========================================================================
 __global__ __launch_bounds__(1024) void k_bin_scan(const RepDev *__restrict__ reps, int nrep) {
-    __shared__ int sh[1024];
-    __shared__ int carry;
+    __shared__ int wsum[16];
     const RepDev &R = reps[blockIdx.x];
     const int nt = R.ntiles;
-    const int tid = threadIdx.x;
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nt; base += 1024) {
-        const int t = base + tid;
-        const int c = t < nt ? R.tcount[t] : 0;
-        sh[tid] = c;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int v = tid >= off ? sh[tid - off] : 0;
-            __syncthreads();
-            sh[tid] += v;
-            __syncthreads();
-        }
-        if (t < nt) {
-            R.tstart[t] = carry + sh[tid] - c;
-            R.tcount[t] = 0;
-            R.tcursor[t] = 0;
-        }
-        __syncthreads();
-        if (tid == 1023) carry += sh[1023];
-        __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int per = (nt + 1023) >> 10;
+    const int beg = min(tid * per, nt), end = min(beg + per, nt);
+    int local = 0;
+    for (int t = beg; t < end; t++) local += R.tcount[t];
+    // block-wide exclusive scan of `local`
+    int inc = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
     }
-    if (tid == 0) R.tstart[nt] = carry;
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    if (wv == 0) {
+        int v = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            int y = __shfl_up(v, o, 64);
+            if (lane >= o) v += y;
+        }
+        if (lane < 16) wsum[lane] = v;
+    }
+    __syncthreads();
+    int run = inc - local + (wv > 0 ? wsum[wv - 1] : 0);
+    for (int t = beg; t < end; t++) {
+        const int c = R.tcount[t];
+        R.tstart[t] = run;
+        run += c;
+        R.tcount[t] = 0;
+        R.tcursor[t] = 0;
+    }
+    if (tid == 1023) R.tstart[nt] = wsum[15];
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(const RepDev *__restrict__ reps, int nrep,
@@ -147,11 +187,17 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const RepDev *__restrict__ 
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
     const int b = (int)(g - start[r]);
-    const int t = R.key[b];
-    const int pos = R.tstart[t] + atomicAdd(&R.tcursor[t], 1);
-    R.permu[pos] = b;
+    int tiles[kMaxTilesPerBead];
+    const int nt = bead_tiles(R, R.sg[3 * b], R.sg[3 * b + 1], R.sg[3 * b + 2], tiles);
+    for (int q = 0; q < nt; q++) {
+        const int t = tiles[q];
+        const int pos = R.tstart[t] + atomicAdd(&R.tcursor[t], 1);
+        R.permu[pos] = b;
+        R.etile[pos] = t;
+    }
 }
 
+// thread per entry slot (27 per bead); slots beyond the filled total exit
 __global__ __launch_bounds__(256) void k_bin_rank(const RepDev *__restrict__ reps, int nrep,
                                                   const long long *__restrict__ start,
                                                   long long total) {
@@ -159,213 +205,130 @@ __global__ __launch_bounds__(256) void k_bin_rank(const RepDev *__restrict__ rep
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
-    const int b = (int)(g - start[r]);
-    const int t = R.key[b];
+    const int q = (int)(g - start[r]);
+    if (q >= R.tstart[R.ntiles]) return;
+    const int t = R.etile[q];
+    const int b = R.permu[q];
     const int st = R.tstart[t], en = R.tstart[t + 1];
     int rank = 0;
-    for (int q = st; q < en; q++) rank += (R.permu[q] < b) ? 1 : 0;
+    for (int e = st; e < en; e++) rank += (R.permu[e] < b) ? 1 : 0;
     R.perm[st + rank] = b;
 }
 
 // ===========================================================================
-// Spread (Lambda): deterministic owner-computes reduction per tile.
-// Semantics of cuda/SELM_Kernel.cu:424-470: each bead adds -t_f * w to the
-// 4x4x4 stencil of each staggered component, w = ((1*phi_z)*phi_y)*phi_x.
+// Spread (Lambda) + viscous predictor, workgroup per tile.
+// Spread semantics of cuda/SELM_Kernel.cu:424-470: each bead adds -t_f * w
+// to the 4x4x4 stencil of each staggered component, w = ((1*phi_z)*phi_y)*phi_x.
+// Predictor semantics of cuda/SELM_Kernel.cu:155-195:
+// u* = u + (mu * L7(u) * dt + f) / rho, L7 the periodic 7-point Laplacian / dx^2.
+// Each cell's spread sum runs over the tile's bucket in ascending bead index,
+// so the result does not depend on scheduling or on other replicas.
 // ===========================================================================
 template <typename T>
-struct BeadRec {
-    int p[3][3];     // [comp][axis] first stencil index, wrapped
-    T w[3][3][4];    // [comp][axis][offset] 1-D factors
-    T f[3];          // -t_f per comp
+struct Acc;
+template <>
+struct Acc<double> {  // plain FP64 sum
+    static __device__ void add(double *s, double *, double x) { *s += x; }
+    static __device__ double value(const double *s, const double *) { return *s; }
 };
-
-template <typename T>
-struct KahanAcc {
-    T s, c;
-    __device__ void init() { s = (T)0; c = (T)0; }
-    __device__ void add(T x) {
-        if constexpr (sizeof(T) == 4) {
-            // Neumaier's compensated summation
-            T t = s + x;
-            if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
-            else c += (x - t) + s;
-            s = t;
-        } else {
-            s += x;
-        }
+template <>
+struct Acc<float> {  // Neumaier-compensated FP32 sum
+    static __device__ void add(float *s, float *c, float x) {
+        float a = *s, t = a + x;
+        if (fabsf(a) >= fabsf(x)) *c += (a - t) + x;
+        else *c += (x - t) + a;
+        *s = t;
     }
-    __device__ T value() const { return s + c; }
+    static __device__ float value(const float *s, const float *c) { return *s + *c; }
 };
 
-constexpr int kSpreadChunk = 128;
+constexpr int kTileCells = 512;  // tile edge <= 8
+constexpr int kSpreadChunk = 64;
 constexpr int kSpreadThreads = 256;
 
 template <typename T>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps,
-                                                           int nrep,
-                                                           const long long *__restrict__ start,
-                                                           long long total) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    BeadRec<T> *rec = (BeadRec<T> *)smem;
-    __shared__ int nb_start[27], nb_cnt[27], nb_pre[28];
-    __shared__ int nnb;
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_ustar(const RepDev *__restrict__ reps,
+                                                                 int nrep,
+                                                                 const long long *__restrict__ start,
+                                                                 long long total) {
+    __shared__ T acc_s[3][kTileCells];
+    __shared__ T acc_c[3][kTileCells];
+    __shared__ SpreadRec<T> rec[kSpreadChunk];
 
     const long long g = blockIdx.x;
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
     const int t = (int)(g - start[r]);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
+    const int ox = tx * R.te[0], oy = ty * R.te[1], oz = tz * R.te[2];
+    const int ex = min(R.te[0], nx - ox), ey = min(R.te[1], ny - oy), ez = min(R.te[2], nz - oz);
+    const int ncell = ex * ey * ez;
 
-    if (tid == 0) {
-        // unique neighbour tiles per axis, fixed order
-        int lx[3], ly[3], lz[3], cx = 0, cy = 0, cz = 0;
-        for (int o = -1; o <= 1; o++) {
-            int a = wrap_idx(tx + o, R.tn[0]);
-            bool dup = false;
-            for (int q = 0; q < cx; q++) dup |= lx[q] == a;
-            if (!dup) lx[cx++] = a;
-            a = wrap_idx(ty + o, R.tn[1]);
-            dup = false;
-            for (int q = 0; q < cy; q++) dup |= ly[q] == a;
-            if (!dup) ly[cy++] = a;
-            a = wrap_idx(tz + o, R.tn[2]);
-            dup = false;
-            for (int q = 0; q < cz; q++) dup |= lz[q] == a;
-            if (!dup) lz[cz++] = a;
+    for (int c = tid; c < 3 * kTileCells; c += kSpreadThreads) {
+        (&acc_s[0][0])[c] = (T)0;
+        (&acc_c[0][0])[c] = (T)0;
+    }
+    const int st = R.tstart[t], en = R.tstart[t + 1];
+    const SpreadRec<T> *brec = (const SpreadRec<T> *)R.brec;
+    constexpr int W = sizeof(SpreadRec<T>) / 4;
+    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
+    for (int base = st; base < en; base += kSpreadChunk) {
+        const int nq = min(kSpreadChunk, en - base);
+        __syncthreads();  // previous chunk consumed / accumulators cleared
+        for (int idx = tid; idx < nq * W; idx += kSpreadThreads) {
+            const int k = idx / W, wd = idx - k * W;
+            ((uint32_t *)&rec[k])[wd] = ((const uint32_t *)&brec[R.perm[base + k]])[wd];
         }
-        int q = 0, acc = 0;
-        for (int c = 0; c < cz; c++)
-            for (int bq = 0; bq < cy; bq++)
-                for (int a = 0; a < cx; a++) {
-                    int nt = (lz[c] * R.tn[1] + ly[bq]) * R.tn[0] + lx[a];
-                    nb_start[q] = R.tstart[nt];
-                    nb_cnt[q] = R.tstart[nt + 1] - R.tstart[nt];
-                    nb_pre[q] = acc;
-                    acc += nb_cnt[q];
-                    q++;
+        __syncthreads();
+        if (wv < 3) {
+            const int d = wv;
+            for (int k = 0; k < nq; k++) {
+                const SpreadRec<T> &B = rec[k];
+                int cx = B.p[d][0] + li;
+                cx -= cx >= nx ? nx : 0;
+                int cy = B.p[d][1] + lj;
+                cy -= cy >= ny ? ny : 0;
+                int cz = B.p[d][2] + lk;
+                cz -= cz >= nz ? nz : 0;
+                cx -= ox;
+                cy -= oy;
+                cz -= oz;
+                if ((unsigned)cx < (unsigned)ex && (unsigned)cy < (unsigned)ey && (unsigned)cz < (unsigned)ez) {
+                    const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
+                    const int cell = (cz * ey + cy) * ex + cx;
+                    Acc<T>::add(&acc_s[d][cell], &acc_c[d][cell], B.f[d] * w);
                 }
-        nb_pre[q] = acc;
-        nnb = q;
+            }
+        }
     }
     __syncthreads();
-    const int ncand = nb_pre[nnb];
 
-    // owned cells: tid and tid + 256 of the te0 x te1 x te2 tile
-    const int ex = min(R.te[0], nx - tx * R.te[0]);
-    const int ey = min(R.te[1], ny - ty * R.te[1]);
-    const int ez = min(R.te[2], nz - tz * R.te[2]);
-    const int ncell = ex * ey * ez;
-    int cxs[2], cys[2], czs[2];
-    bool own[2];
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-        int c = tid + q * kSpreadThreads;
-        own[q] = c < ncell;
-        int lx = c % ex, ly = (c / ex) % ey, lz = c / (ex * ey);
-        cxs[q] = tx * R.te[0] + lx;
-        cys[q] = ty * R.te[1] + ly;
-        czs[q] = tz * R.te[2] + lz;
-    }
-    KahanAcc<T> acc[2][3];
-#pragma unroll
-    for (int q = 0; q < 2; q++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) acc[q][d].init();
-
-    const int f4 = R.f4;
-    for (int base = 0; base < ncand; base += kSpreadChunk) {
-        const int nq = min(kSpreadChunk, ncand - base);
-        if (tid < nq) {
-            const int cq = base + tid;
-            int s = 0;
-            while (nb_pre[s + 1] <= cq) s++;
-            const int b = R.perm[nb_start[s] + (cq - nb_pre[s])];
-            BeadRec<T> &B = rec[tid];
-            const double sg0 = R.sg[3 * b], sg1 = R.sg[3 * b + 1], sg2 = R.sg[3 * b + 2];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                B.f[d] = (T)(-R.tf[3 * b + d]);
-#pragma unroll
-                for (int e = 0; e < 3; e++) {
-                    double re = (e == 0 ? sg0 : (e == 1 ? sg1 : sg2)) - (d != e ? 0.5 : 0.0);
-                    int p = (int)(floor(re) - 1.0);
-                    int ne = e == 0 ? nx : (e == 1 ? ny : nz);
-                    B.p[d][e] = wrap_idx(p, ne);
-#pragma unroll
-                    for (int o = 0; o < 4; o++) B.w[d][e][o] = (T)peskin_phi(fabs(re - (p + o)), f4);
-                }
-            }
-        }
-        __syncthreads();
-        for (int k = 0; k < nq; k++) {
-            const BeadRec<T> &B = rec[k];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    int ox = cxs[q] - B.p[d][0];
-                    ox += ox < 0 ? nx : 0;
-                    int oy = cys[q] - B.p[d][1];
-                    oy += oy < 0 ? ny : 0;
-                    int oz = czs[q] - B.p[d][2];
-                    oz += oz < 0 ? nz : 0;
-                    if (ox < 4 && oy < 4 && oz < 4) {
-                        T w = (B.w[d][2][oz] * B.w[d][1][oy]) * B.w[d][0][ox];
-                        acc[q][d].add(B.f[d] * w);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    T *fsp = (T *)R.fsp;
-    const long long N = R.N;
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-        if (!own[q]) continue;
-        long long n = ((long long)czs[q] * ny + cys[q]) * nx + cxs[q];
-#pragma unroll
-        for (int d = 0; d < 3; d++) fsp[d * N + n] = acc[q][d].value();
-    }
-}
-
-// ===========================================================================
-// u* = u + (mu * L7(u) * dt + f) / rho  (cuda/SELM_Kernel.cu:155-195), in
-// place over the spread buffer.
-// ===========================================================================
-template <typename T>
-__global__ __launch_bounds__(256) void k_ustar(const RepDev *__restrict__ reps, int nrep,
-                                               const long long *__restrict__ start,
-                                               long long total) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const long long n = g - start[r];
-    const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const long long N = R.N, Nyx = (long long)nx * ny;
-    const int k = (int)(n / Nyx);
-    const int j = (int)((n % Nyx) / nx);
-    const int i = (int)(n % nx);
-    const long long i1 = (i != nx - 1) ? 1 : -i;
-    const long long j1 = (j != ny - 1) ? nx : -(long long)j * nx;
-    const long long k1 = (k != nz - 1) ? Nyx : -(long long)k * Nyx;
-    const long long i0 = (i != 0) ? -1 : nx - 1;
-    const long long j0 = (j != 0) ? -(long long)nx : (long long)(ny - 1) * nx;
-    const long long k0 = (k != 0) ? -Nyx : (long long)(nz - 1) * Nyx;
     const T dxsq = (T)(R.dx * R.dx);
     const T mu = (T)R.mu, dt = (T)R.dt, rho = (T)R.rho;
     const T *u = (const T *)R.u;
-    T *f = (T *)R.fsp;
+    T *out = (T *)R.fsp;
+    for (int c = tid; c < ncell; c += kSpreadThreads) {
+        const int cx = c % ex, cy = (c / ex) % ey, cz = c / (ex * ey);
+        const int i = ox + cx, j = oy + cy, k = oz + cz;
+        const long long n = (long long)k * Nyx + (long long)j * nx + i;
+        const long long i1 = (i != nx - 1) ? 1 : -i;
+        const long long j1 = (j != ny - 1) ? nx : -(long long)j * nx;
+        const long long k1 = (k != nz - 1) ? Nyx : -(long long)k * Nyx;
+        const long long i0 = (i != 0) ? -1 : nx - 1;
+        const long long j0 = (j != 0) ? -(long long)nx : (long long)(ny - 1) * nx;
+        const long long k0 = (k != 0) ? -Nyx : (long long)(nz - 1) * Nyx;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const T *ud = u + d * N;
-        T uc = ud[n];
-        T lap = (ud[n + i1] + ud[n + i0] + ud[n + j1] + ud[n + j0] + ud[n + k1] + ud[n + k0] - (T)6 * uc) / dxsq;
-        f[d * N + n] = uc + (mu * lap * dt + f[d * N + n]) / rho;
+        for (int d = 0; d < 3; d++) {
+            const T *ud = u + d * N;
+            const T uc = ud[n];
+            const T lap = (ud[n + i1] + ud[n + i0] + ud[n + j1] + ud[n + j0] + ud[n + k1] + ud[n + k0] - (T)6 * uc) / dxsq;
+            const T f = Acc<T>::value(&acc_s[d][c], &acc_c[d][c]);
+            out[d * N + n] = uc + (mu * lap * dt + f) / rho;
+        }
     }
 }
 
@@ -667,6 +630,7 @@ __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ 
     R.fc[3 * b + 2] = f2 + R.fext[3 * b + 2];
 }
 
+
 template <typename T>
 __global__ void k_fill(T *p, long long n, T v) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -679,127 +643,80 @@ __global__ void k_fill(T *p, long long n, T v) {
 static inline unsigned nblk(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 template <typename T>
-void launch_step_t(const StepLaunch &L, hipStream_t st) {
+static void launch_fft_generic(const StepLaunch &L, int stage, hipStream_t st);
+
+template <typename T>
+static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
     const RepDev *reps = L.reps;
     const int nrep = L.nrep;
-    if (L.tot_bead > 0) {
-        hipLaunchKernelGGL(k_bead_update<T>, dim3(nblk(L.tot_bead, 4)), dim3(256), 0, st, reps, nrep,
-                           L.st_bead, L.tot_bead);
-    }
-    hipLaunchKernelGGL(k_bin_scan, dim3(nrep), dim3(1024), 0, st, reps, nrep);
-    if (L.tot_bead > 0) {
-        hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(L.tot_bead, 256)), dim3(256), 0, st, reps, nrep,
-                           L.st_bead, L.tot_bead);
-        hipLaunchKernelGGL(k_bin_rank, dim3(nblk(L.tot_bead, 256)), dim3(256), 0, st, reps, nrep,
-                           L.st_bead, L.tot_bead);
-    }
-    hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads),
-                       kSpreadChunk * sizeof(BeadRec<T>), st, reps, nrep, L.st_tile, L.tot_tile);
-    hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep, L.st_pts,
-                       L.tot_pts);
-    hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps, nrep,
-                       L.st_fx, L.tot_fx, L.twmax);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3((unsigned)L.tot_fy), dim3(256), L.lds_fy, st, reps, nrep,
-                       L.st_fy, L.tot_fy, -1, L.twmax);
-    hipLaunchKernelGGL(k_fft_z_proj<T>, dim3((unsigned)L.tot_fz), dim3(256), L.lds_fz, st, reps, nrep,
-                       L.st_fz, L.tot_fz, L.twmax);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3((unsigned)L.tot_fy), dim3(256), L.lds_fy, st, reps, nrep,
-                       L.st_fy, L.tot_fy, +1, L.twmax);
-    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps, nrep,
-                       L.st_fx, L.tot_fx, L.twmax);
-    hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
-                       reps, nrep, L.st_bead, L.tot_bead, 1);
-}
-
-void launch_step(const StepLaunch &L, hipStream_t st) {
-    if (L.fp64) launch_step_t<double>(L, st);
-    else launch_step_t<float>(L, st);
-}
-
-void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
-    // one stage of the step, for per-stage timing; same kernels as launch_step
-    const RepDev *reps = L.reps;
-    const int nrep = L.nrep;
-    const bool f64 = L.fp64;
-#define SELM_T(kern) (f64 ? (const void *)kern<double> : (const void *)kern<float>)
     switch (stage) {
         case kStageBead:
-            if (L.tot_bead > 0) {
-                if (f64)
-                    hipLaunchKernelGGL(k_bead_update<double>, dim3(nblk(L.tot_bead, 4)), dim3(256), 0, st, reps,
-                                       nrep, L.st_bead, L.tot_bead);
-                else
-                    hipLaunchKernelGGL(k_bead_update<float>, dim3(nblk(L.tot_bead, 4)), dim3(256), 0, st, reps,
-                                       nrep, L.st_bead, L.tot_bead);
-            }
+            if (L.tot_bead > 0)
+                hipLaunchKernelGGL(k_bead_update<T>, dim3(nblk(L.tot_bead, 4)), dim3(256), 0, st, reps, nrep,
+                                   L.st_bead, L.tot_bead);
             break;
         case kStageBin:
             hipLaunchKernelGGL(k_bin_scan, dim3(nrep), dim3(1024), 0, st, reps, nrep);
             if (L.tot_bead > 0) {
                 hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(L.tot_bead, 256)), dim3(256), 0, st, reps, nrep,
                                    L.st_bead, L.tot_bead);
-                hipLaunchKernelGGL(k_bin_rank, dim3(nblk(L.tot_bead, 256)), dim3(256), 0, st, reps, nrep,
-                                   L.st_bead, L.tot_bead);
+                hipLaunchKernelGGL(k_bin_rank, dim3(nblk(L.tot_ent, 256)), dim3(256), 0, st, reps, nrep,
+                                   L.st_ent, L.tot_ent);
             }
             break;
         case kStageSpread:
-            if (f64)
-                hipLaunchKernelGGL(k_spread<double>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads),
-                                   kSpreadChunk * sizeof(BeadRec<double>), st, reps, nrep, L.st_tile, L.tot_tile);
-            else
-                hipLaunchKernelGGL(k_spread<float>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads),
-                                   kSpreadChunk * sizeof(BeadRec<float>), st, reps, nrep, L.st_tile, L.tot_tile);
-            break;
-        case kStageUstar:
-            if (f64)
-                hipLaunchKernelGGL(k_ustar<double>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep,
-                                   L.st_pts, L.tot_pts);
-            else
-                hipLaunchKernelGGL(k_ustar<float>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep,
-                                   L.st_pts, L.tot_pts);
-            break;
-        case kStageFftX:
-            if (f64)
-                hipLaunchKernelGGL(k_fft_x_r2c<double>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps,
-                                   nrep, L.st_fx, L.tot_fx, L.twmax);
-            else
-                hipLaunchKernelGGL(k_fft_x_r2c<float>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps,
-                                   nrep, L.st_fx, L.tot_fx, L.twmax);
-            break;
-        case kStageFftY:
-        case kStageFftYi: {
-            int dir = stage == kStageFftY ? -1 : +1;
-            if (f64)
-                hipLaunchKernelGGL(k_fft_y<double>, dim3((unsigned)L.tot_fy), dim3(256), L.lds_fy, st, reps,
-                                   nrep, L.st_fy, L.tot_fy, dir, L.twmax);
-            else
-                hipLaunchKernelGGL(k_fft_y<float>, dim3((unsigned)L.tot_fy), dim3(256), L.lds_fy, st, reps,
-                                   nrep, L.st_fy, L.tot_fy, dir, L.twmax);
-            break;
-        }
-        case kStageFftZ:
-            if (f64)
-                hipLaunchKernelGGL(k_fft_z_proj<double>, dim3((unsigned)L.tot_fz), dim3(256), L.lds_fz, st,
-                                   reps, nrep, L.st_fz, L.tot_fz, L.twmax);
-            else
-                hipLaunchKernelGGL(k_fft_z_proj<float>, dim3((unsigned)L.tot_fz), dim3(256), L.lds_fz, st,
-                                   reps, nrep, L.st_fz, L.tot_fz, L.twmax);
-            break;
-        case kStageFftXi:
-            if (f64)
-                hipLaunchKernelGGL(k_fft_x_c2r<double>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps,
-                                   nrep, L.st_fx, L.tot_fx, L.twmax);
-            else
-                hipLaunchKernelGGL(k_fft_x_c2r<float>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps,
-                                   nrep, L.st_fx, L.tot_fx, L.twmax);
+            hipLaunchKernelGGL(k_spread_ustar<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads), 0, st, reps,
+                               nrep, L.st_tile, L.tot_tile);
             break;
         case kStageBonds:
-            hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0,
-                               st, reps, nrep, L.st_bead, L.tot_bead, 1);
+            hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
+                               reps, nrep, L.st_bead, L.tot_bead, 1);
+            break;
+        case kStageFftX:
+        case kStageFftY:
+        case kStageFftZ:
+        case kStageFftYi:
+        case kStageFftXi:
+            if (L.fast) launch_fft_fast(L, stage, st);
+            else launch_fft_generic<T>(L, stage, st);
             break;
         default: break;
     }
-#undef SELM_T
+}
+
+template <typename T>
+static void launch_fft_generic(const StepLaunch &L, int stage, hipStream_t st) {
+    const RepDev *reps = L.reps;
+    const int nrep = L.nrep;
+    switch (stage) {
+        case kStageFftX:
+            hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps, nrep,
+                               L.st_fx, L.tot_fx, L.twmax);
+            break;
+        case kStageFftY:
+        case kStageFftYi:
+            hipLaunchKernelGGL(k_fft_y<T>, dim3((unsigned)L.tot_fy), dim3(256), L.lds_fy, st, reps, nrep,
+                               L.st_fy, L.tot_fy, stage == kStageFftY ? -1 : +1, L.twmax);
+            break;
+        case kStageFftZ:
+            hipLaunchKernelGGL(k_fft_z_proj<T>, dim3((unsigned)L.tot_fz), dim3(256), L.lds_fz, st, reps, nrep,
+                               L.st_fz, L.tot_fz, L.twmax);
+            break;
+        case kStageFftXi:
+            hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3((unsigned)L.tot_fx), dim3(256), L.lds_fx, st, reps, nrep,
+                               L.st_fx, L.tot_fx, L.twmax);
+            break;
+        default: break;
+    }
+}
+
+void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
+    if (L.fp64) launch_stage_t<double>(L, stage, st);
+    else launch_stage_t<float>(L, stage, st);
+}
+
+void launch_step(const StepLaunch &L, hipStream_t st) {
+    for (int s = 0; s < kNumStages; s++) launch_stage(L, s, st);
 }
 
 void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st) {
@@ -813,5 +730,7 @@ void launch_fill_f64(double *p, long long n, double v, hipStream_t st) {
 void launch_fill_f32(float *p, long long n, float v, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
 }
+
+size_t spread_rec_bytes(bool fp64) { return fp64 ? sizeof(SpreadRec<double>) : sizeof(SpreadRec<float>); }
 
 }  // namespace selm

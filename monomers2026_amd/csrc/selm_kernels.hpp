@@ -10,8 +10,7 @@ namespace selm {
 enum Stage {
     kStageBead = 0,
     kStageBin,
-    kStageSpread,
-    kStageUstar,
+    kStageSpread,  // spread + viscous predictor u*
     kStageFftX,
     kStageFftY,
     kStageFftZ,
@@ -26,10 +25,11 @@ struct StepLaunch {
     const RepDev *reps;   // device array [nrep]
     int nrep;
     bool fp64;
-    const long long *st_bead, *st_tile, *st_pts, *st_fx, *st_fy, *st_fz;  // device [nrep+1]
-    long long tot_bead, tot_tile, tot_pts, tot_fx, tot_fy, tot_fz;
+    const long long *st_bead, *st_tile, *st_ent, *st_fx, *st_fy, *st_fz;  // device [nrep+1]
+    long long tot_bead, tot_tile, tot_ent, tot_fx, tot_fy, tot_fz;
     size_t lds_fx, lds_fy, lds_fz;   // dynamic LDS bytes per pass
     int twmax;                       // LDS twiddle slots (elements) preceding the data
+    bool fast;                       // compile-time-length FFT kernels (selm_fft_kernels.hip)
 };
 
 void launch_step(const StepLaunch &L, hipStream_t st);
@@ -37,5 +37,12 @@ void launch_stage(const StepLaunch &L, int stage, hipStream_t st);
 void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st);
 void launch_fill_f64(double *p, long long n, double v, hipStream_t st);
 void launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+size_t spread_rec_bytes(bool fp64);
+
+// selm_fft_kernels.hip
+bool fft_fast_supported(const int n[3]);
+void fft_fast_blocks(bool fp64, int *bx, int *by, int *bz);
+void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, size_t *lz);
+void launch_fft_fast(const StepLaunch &L, int stage, hipStream_t st);
 
 }  // namespace selm

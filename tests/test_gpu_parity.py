@@ -71,17 +71,19 @@ def test_chains1000_64_fp64(gpu_lib, noise):
         _compare(sysg, o, FP64_TOL, "chains1000_64 noise=%d step=%d" % (noise, o.timestep), box=64.0)
 
 
-def test_random_fluid_one_step_fp64(gpu_lib):
-    """One step from a random divergence-free-free (arbitrary) u^n on a
-    non-cubic mixed-radix grid: exercises gather, spread, u*, all FFT radices
-    and the projection against the oracle's real-space stencils."""
+@pytest.mark.parametrize("dims", [(30, 24, 20), (36, 25, 18)])
+def test_random_fluid_one_step_fp64(gpu_lib, dims):
+    """Steps from a random (not divergence-free) u^n on non-cubic mixed-radix
+    grids: gather, spread, u*, the FFT radices and the projection against the
+    oracle's real-space stencils.  (30, 24, 20) runs the compile-time-length
+    FFT kernels; (36, 25, 18) has no specialised length and runs the generic
+    kernels, with ragged 7-cell tiles along y."""
     import oracle_lib as O
     from monomers2026_amd import selm
     rng = np.random.default_rng(5)
-    dims = (30, 24, 20)
-    lo, hi = (0.0, 0.0, 0.0), (30.0, 24.0, 20.0)
+    lo, hi = (0.0, 0.0, 0.0), tuple(float(d) for d in dims)
     M = 40
-    x = rng.uniform(0, 20, (M, 3))
+    x = rng.uniform(0, min(dims), (M, 3))
     v = rng.standard_normal((M, 3))
     m = rng.uniform(0.5, 2.0, M)
     ij = np.array([(i, i + 1) for i in range(M - 1)], np.int32)
@@ -103,10 +105,10 @@ def test_random_fluid_one_step_fp64(gpu_lib):
         o.set_field(0, u0)
         s.step(1)
         o.step(1)
-        _compare(s, o, FP64_TOL, "random-fluid 30x24x20 noise=%d" % noise, box=30.0)
+        _compare(s, o, FP64_TOL, "random-fluid %s noise=%d" % (dims, noise), box=float(dims[0]))
         s.step(3)
         o.step(3)
-        _compare(s, o, FP64_TOL, "random-fluid 30x24x20 noise=%d step 4" % noise, box=30.0)
+        _compare(s, o, FP64_TOL, "random-fluid %s noise=%d step 4" % (dims, noise), box=float(dims[0]))
         s.close()
 
 
@@ -146,6 +148,25 @@ def test_fp32_bulk_displacement(gpu_lib):
         err = np.abs(dg - do).max() / np.abs(do).max()
         print("fp32-bulk chains1000_64 step %d: displacement rel err %.2e" % (o.timestep, err), flush=True)
         assert err < 1e-5
+
+
+def test_fast_vs_generic_fft_fp32(gpu_lib, monkeypatch):
+    """The compile-time-length FFT kernels and the generic ones agree (FP32)."""
+    from monomers2026_amd import selm, workloads
+    w = workloads.chains1000_64()
+    out = []
+    for generic in (False, True):
+        if generic:
+            monkeypatch.setenv("SELM_FFT_GENERIC", "1")
+        s = w.build_system(selm.PREC_FP32_BULK, True, seed=5)
+        s.step(10)
+        out.append((s.beads()[1], s.field()))
+        s.close()
+    monkeypatch.delenv("SELM_FFT_GENERIC", raising=False)
+    ev = _rel(out[0][0], out[1][0])
+    eu = _rel(out[0][1], out[1][1])
+    print("fast vs generic FFT fp32: v %.2e u %.2e" % (ev, eu), flush=True)
+    assert ev < 1e-5 and eu < 1e-5
 
 
 def test_deterministic_repeat(gpu_lib):
