@@ -63,11 +63,11 @@ def stage_bytes(w, s):
     N = w.edge ** 3
     Nc = (w.edge // 2 + 1) * w.edge * w.edge
     M = w.M
+    rec = 208 if s == 4 else 368  # spread record written per bead
     return {
-        "bead_update": M * (192 * s + 24 + 152),
-        "binning": M * 4 * 6,
-        "spread_ustar": 6 * N * s + M * 3.4 * (208 if s == 4 else 368),
-        "fft_x": 3 * N * s + 3 * Nc * 2 * s,
+        "bead_update": M * (192 * s + 24 + 152 + rec),
+        "spread": M * (24 + 3 * s) + 192 * M * s,
+        "ustar_fft_x": 3 * N * s + 3 * Nc * 2 * s,
         "fft_y": 2 * 3 * Nc * 2 * s,
         "fft_z_projection": 2 * 3 * Nc * 2 * s,
         "ifft_y": 2 * 3 * Nc * 2 * s,

@@ -40,7 +40,7 @@ extern "C" {
 #define SELM_PREC_FP32_BULK 1
 
 /* per-stage indices for selm_stage_times */
-#define SELM_NUM_STAGES 9
+#define SELM_NUM_STAGES 8
 
 typedef struct selm_ctx selm_ctx;
 typedef struct selm_pool selm_pool;
@@ -138,8 +138,8 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
 
 /* Run nsteps steps stage by stage with HIP events around each stage on the
  * ctx stream; ms[s] receives the summed milliseconds of stage s
- * (0 bead update, 1 binning, 2 spread + viscous u*, 3 FFT x, 4 FFT y,
- *  5 FFT z + projection, 6 inverse FFT y, 7 inverse FFT x, 8 bond forces). */
+ * (0 bead update + binning, 1 spread, 2 viscous u* + FFT x, 3 FFT y,
+ *  4 FFT z + projection, 5 inverse FFT y, 6 inverse FFT x, 7 bond forces). */
 int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
 
 /* Sum over the grid of u* for each component before the k=0 mode is

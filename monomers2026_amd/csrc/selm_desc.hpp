@@ -10,7 +10,8 @@ namespace selm {
 
 // Fluid arrays of one replica (T = double or float):
 //   u     [3][nz][ny][nx]   velocity u^n (MAC faces: u_x at (i, j+1/2, k+1/2) dx)
-//   fsp   [3][nz][ny][nx]   spread impulse, then u* in place
+//   fsp   [3][nz][ny][nx]   u* (the FFT input) on the runtime-length FFT path
+//   pool  [slot][3][tile]   spread impulse of the tiles that hold beads
 //   spec  [3][nz][ny][xc]   half spectrum (xc = nx/2+1), unnormalised DFT
 // Bead arrays (always FP64, AoS [M][3] like LAMMPS atom->x):
 //   x v fc ftot fext tf sg ; m [M]
@@ -20,6 +21,8 @@ struct RepDev {
     int xc;               // nx/2 + 1
     long long N, Nc;      // nx*ny*nz, xc*ny*nz
     void *u, *fsp, *spec;
+    void *pool;           // spread impulse per tile: [tile][3][te0*te1*te2] of T, valid when
+    int *tstamp;          // tstamp[tile] == the current step (a bead touched the tile)
     const void *tw[3];    // cplx<T>[n_a]: e^{-2 pi i m / n_a}
     const void *ktab[3];  // per axis, T[4*n_a]: {lam_a, Dre_a, Dim_a, sqrt-part}
     RadixPlan plan[3];
@@ -32,8 +35,10 @@ struct RepDev {
     int M;
     double *x, *v, *fc, *ftot, *m, *fext, *tf, *sg;
     void *brec;                // SpreadRec<T>[M]: per-bead stencil record for the spread
-    int *tcount, *tstart, *tcursor;   // per tile: bucket counts, starts (ntiles+1), fill cursors
-    int *permu, *etile, *perm;        // bucket entries [27*M]: unordered, their tile, ordered
+    int *bcnt;                 // per tile: beads inserted this step (may exceed kBucketCap)
+    int *bucket;               // [ntiles][kBucketCap] bead indices, unordered
+    int *ovf;                  // [27*M][2] (tile, bead) pairs beyond a full bucket
+    int *ctr;                  // [8]: overflow count per step parity [0,1], status flags [4]
     int *adj_ptr, *adj_i1, *adj_i2;   // CSR over beads: bonds touching each bead, bond order
     double *adj_K, *adj_r0;
     // ---- scalars
@@ -64,6 +69,15 @@ struct SpreadRec {
 // of the three staggered stencils spans cells floor(s-1/2)-1 .. floor(s)+2,
 // which meets at most 3 tiles (2 unless a ragged tile is narrower than 4).
 constexpr int kMaxTilesPerBead = 27;
+constexpr int kBucketCap = 256;
+constexpr int kStatusListOverflow = 1;  // a tile held more beads than the spread can order
+
+// Blocks of a launch are dealt round-robin to the 8 XCDs (each with its own
+// L2); remap so that consecutive logical blocks run on the same XCD and share
+// cache lines.  G is the (padded) grid size, a multiple of 8.
+__device__ __forceinline__ long long xcd_remap(long long b, long long G) {
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
 
 // Work table for one stage: item g of the launch belongs to replica r with
 // start[r] <= g < start[r+1].

@@ -116,7 +116,9 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     b += al(3 * (size_t)G.Nc * 2 * s);    // spec
     for (int a = 0; a < 3; a++) b += al((size_t)G.n[a] * 2 * s) + al(4 * (size_t)G.n[a] * s);
     b += al(8) + al(3 * 8);               // ts, diag
-    b += 2 * al((size_t)G.ntiles * 4) + al(((size_t)G.ntiles + 1) * 4);
+    const size_t tcells = (size_t)G.te[0] * G.te[1] * G.te[2];
+    b += al((size_t)G.ntiles * 3 * tcells * s);                   // pool
+    b += 2 * al((size_t)G.ntiles * 4) + al((size_t)G.ntiles * kBucketCap * 4) + al(8 * 4);
     return b;
 }
 
@@ -125,7 +127,7 @@ static size_t bead_bytes(int M, int nadj, bool fp64) {
     b += 7 * al(3 * (size_t)M * 8);
     b += al((size_t)M * 8);
     b += al((size_t)M * spread_rec_bytes(fp64));
-    b += 3 * al((size_t)kMaxTilesPerBead * M * 4);
+    b += al((size_t)kMaxTilesPerBead * M * 8);
     b += al(((size_t)M + 1) * 4);
     b += 2 * al((size_t)nadj * 4) + 2 * al((size_t)nadj * 8);
     return b;
@@ -168,9 +170,12 @@ static void carve_grid(Replica &R, char *base) {
     }
     R.D.ts = (long long *)take(8);
     R.D.diag = (double *)take(3 * 8);
-    R.D.tcount = (int *)take((size_t)G.ntiles * 4);
-    R.D.tcursor = (int *)take((size_t)G.ntiles * 4);
-    R.D.tstart = (int *)take(((size_t)G.ntiles + 1) * 4);
+    const size_t tcells = (size_t)G.te[0] * G.te[1] * G.te[2];
+    R.D.pool = take((size_t)G.ntiles * 3 * tcells * s);
+    R.D.tstamp = (int *)take((size_t)G.ntiles * 4);
+    R.D.bcnt = (int *)take((size_t)G.ntiles * 4);
+    R.D.bucket = (int *)take((size_t)G.ntiles * kBucketCap * 4);
+    R.D.ctr = (int *)take(8 * 4);
 }
 
 static void carve_beads(Replica &R, char *base) {
@@ -192,9 +197,7 @@ static void carve_beads(Replica &R, char *base) {
     R.D.sg = (double *)take(3 * (size_t)M * 8);
     R.D.m = (double *)take((size_t)M * 8);
     R.D.brec = take((size_t)M * spread_rec_bytes(R.fp64));
-    R.D.permu = (int *)take((size_t)kMaxTilesPerBead * M * 4);
-    R.D.etile = (int *)take((size_t)kMaxTilesPerBead * M * 4);
-    R.D.perm = (int *)take((size_t)kMaxTilesPerBead * M * 4);
+    R.D.ovf = (int *)take((size_t)kMaxTilesPerBead * M * 8);
     R.D.adj_ptr = (int *)take(((size_t)M + 1) * 4);
     R.D.adj_i1 = (int *)take(nadj * 4);
     R.D.adj_i2 = (int *)take(nadj * 4);
@@ -392,8 +395,9 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
     const size_t s = R.fp64 ? 8 : 4;
     HIPCHK(hipMemsetAsync(R.D.u, 0, 3 * (size_t)R.G.N * s, st));
     HIPCHK(hipMemsetAsync(R.D.fsp, 0, 3 * (size_t)R.G.N * s, st));
-    HIPCHK(hipMemsetAsync(R.D.tcount, 0, (size_t)R.G.ntiles * 4, st));
-    HIPCHK(hipMemsetAsync(R.D.tcursor, 0, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.bcnt, 0, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.tstamp, 0xff, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.ctr, 0, 8 * 4, st));
     HIPCHK(hipMemsetAsync(R.D.ts, 0, 8, st));
     HIPCHK(hipMemsetAsync(R.D.diag, 0, 24, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -471,7 +475,7 @@ struct Batch {
             h[r].bz = G.bz;
             sb[r + 1] = sb[r] + R.M;
             stl[r + 1] = stl[r] + G.ntiles;
-            sen[r + 1] = sen[r] + (long long)kMaxTilesPerBead * R.M;
+            sen[r + 1] = sen[r] + G.N;
             long long rows = 3LL * G.n[1] * G.n[2];
             sfx[r + 1] = sfx[r] + (rows + G.bx - 1) / G.bx;
             sfy[r + 1] = sfy[r] + 3LL * G.n[2] * ((G.xc + G.by - 1) / G.by);
@@ -503,13 +507,13 @@ struct Batch {
         L.fp64 = fp64;
         L.st_bead = d_tabs;
         L.st_tile = d_tabs + (nrep + 1);
-        L.st_ent = d_tabs + 2 * (nrep + 1);
+        L.st_pts = d_tabs + 2 * (nrep + 1);
         L.st_fx = d_tabs + 3 * (nrep + 1);
         L.st_fy = d_tabs + 4 * (nrep + 1);
         L.st_fz = d_tabs + 5 * (nrep + 1);
         L.tot_bead = sb[nrep];
         L.tot_tile = stl[nrep];
-        L.tot_ent = sen[nrep];
+        L.tot_pts = sen[nrep];
         L.tot_fx = sfx[nrep];
         L.tot_fy = sfy[nrep];
         L.tot_fz = sfz[nrep];

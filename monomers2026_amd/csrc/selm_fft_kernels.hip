@@ -2,7 +2,8 @@
 // (compile-time length) path.  Each kernel switches on the replica's edge
 // length per workgroup, so one launch still covers a heterogeneous batch.
 //
-//   k_fx_r2c    rows along x: two real rows packed as one complex sequence
+//   k_fx_r2c    rows along x of u* (evaluated on load from u^n and the sparse
+//               spread): two real rows packed as one complex sequence
 //   k_fy        columns along y (forward or inverse), in place on spec
 //   k_fz_proj   columns along z: forward DFT, thermal forcing in k-space,
 //               Helmholtz projection, k=0 removal, 1/N, inverse DFT
@@ -10,6 +11,7 @@
 //
 // Spectrum layout: spec[d][z][y][xc], xc = nx/2+1, unnormalised DFT.
 #include "selm_fft.hpp"
+#include "selm_fluid.hpp"
 #include "selm_kernels.hpp"
 
 namespace selm {
@@ -18,8 +20,8 @@ template <typename T>
 struct FastCfg;
 template <>
 struct FastCfg<float> {
-    static constexpr int BX = 32, BY = 16, BZ = 16;          // rows / columns per workgroup
-    static constexpr int NTX = 256, NTY = 256, NTZ = 512;     // threads per workgroup
+    static constexpr int BX = 32, BY = 16, BZ = 8;           // rows / columns per workgroup
+    static constexpr int NTX = 256, NTY = 256, NTZ = 256;     // threads per workgroup
 };
 template <>
 struct FastCfg<double> {
@@ -38,13 +40,23 @@ template <typename T, int N>
 __device__ __forceinline__ void fx_r2c_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
     constexpr int BX = FastCfg<T>::BX, P = BX / 2, S = P + 1, NT = FastCfg<T>::NTX;
     constexpr int XC = N / 2 + 1;
-    const long long rows = 3LL * R.n[1] * R.n[2];
+    const int ny = R.n[1], nz = R.n[2];
+    const long long rows = 3LL * ny * nz;
     const long long row0 = item * BX;
-    const T *in = (const T *)R.fsp;
+    const int step = (int)(R.ts[0] + 1);
+    // rows (d, z, y) of u*, evaluated on load from u^n and the sparse spread
     for (int idx = threadIdx.x; idx < BX * N; idx += NT) {
         const int rl = idx / N, e = idx - rl * N;
         const long long row = row0 + rl;
-        const T v = row < rows ? in[row * N + e] : (T)0;
+        T v = (T)0;
+        if (row < rows) {
+            const int y = (int)(row % ny);
+            const long long dzr = row / ny;
+            const int z = (int)(dzr % nz), d = (int)(dzr / nz);
+            int tyz, lyz;
+            spread_row(R, y, z, tyz, lyz);
+            v = ustar_at<T>(R, d, e, y, z, tyz, lyz, step);
+        }
         ((T *)&a[e * S + (rl >> 1)])[rl & 1] = v;
     }
     __syncthreads();
@@ -211,7 +223,7 @@ __global__ __launch_bounds__(FastCfg<T>::NTX) void k_fx_r2c(const RepDev *__rest
                                                             const long long *__restrict__ start,
                                                             long long total, int twmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = blockIdx.x;
+    const long long g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
@@ -230,7 +242,7 @@ __global__ __launch_bounds__(FastCfg<T>::NTX) void k_fx_c2r(const RepDev *__rest
                                                             const long long *__restrict__ start,
                                                             long long total, int twmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = blockIdx.x;
+    const long long g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
@@ -249,7 +261,7 @@ __global__ __launch_bounds__(FastCfg<T>::NTY) void k_fy(const RepDev *__restrict
                                                         const long long *__restrict__ start, long long total,
                                                         int dir, int twmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = blockIdx.x;
+    const long long g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
@@ -269,7 +281,7 @@ __global__ __launch_bounds__(FastCfg<T>::NTZ) void k_fz_proj(const RepDev *__res
                                                              const long long *__restrict__ start,
                                                              long long total, int twmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = blockIdx.x;
+    const long long g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
@@ -317,24 +329,26 @@ void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, 
     *lz = ((size_t)twmax + (size_t)3 * BZ * n[2]) * csz;
 }
 
+static inline unsigned pad8(long long n) { return (unsigned)((n + 7) & ~7LL); }
+
 template <typename T>
 static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
     switch (stage) {
         case kStageFftX:
-            hipLaunchKernelGGL(k_fx_r2c<T>, dim3((unsigned)L.tot_fx), dim3(FastCfg<T>::NTX), L.lds_fx, st, L.reps,
+            hipLaunchKernelGGL(k_fx_r2c<T>, dim3(pad8(L.tot_fx)), dim3(FastCfg<T>::NTX), L.lds_fx, st, L.reps,
                                L.nrep, L.st_fx, L.tot_fx, L.twmax);
             break;
         case kStageFftY:
         case kStageFftYi:
-            hipLaunchKernelGGL(k_fy<T>, dim3((unsigned)L.tot_fy), dim3(FastCfg<T>::NTY), L.lds_fy, st, L.reps,
+            hipLaunchKernelGGL(k_fy<T>, dim3(pad8(L.tot_fy)), dim3(FastCfg<T>::NTY), L.lds_fy, st, L.reps,
                                L.nrep, L.st_fy, L.tot_fy, stage == kStageFftY ? -1 : +1, L.twmax);
             break;
         case kStageFftZ:
-            hipLaunchKernelGGL(k_fz_proj<T>, dim3((unsigned)L.tot_fz), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
+            hipLaunchKernelGGL(k_fz_proj<T>, dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
                                L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
             break;
         case kStageFftXi:
-            hipLaunchKernelGGL(k_fx_c2r<T>, dim3((unsigned)L.tot_fx), dim3(FastCfg<T>::NTX), L.lds_fx, st, L.reps,
+            hipLaunchKernelGGL(k_fx_c2r<T>, dim3(pad8(L.tot_fx)), dim3(FastCfg<T>::NTX), L.lds_fx, st, L.reps,
                                L.nrep, L.st_fx, L.tot_fx, L.twmax);
             break;
         default: break;

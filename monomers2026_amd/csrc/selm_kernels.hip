@@ -4,25 +4,21 @@
 // launch advances any number of heterogeneous replicas (a single system is a
 // batch of one).  Per step, in order:
 //
-//   k_bead_update     wave per bead: 4x4x4 Peskin gather of u^n for the three
-//                     staggered components (64 lanes = 64 stencil points,
-//                     wave reduction), drag + thermal kick, x/v/f update; the
-//                     bead's spread record; tile counts for binning
-//   k_bin_scan        workgroup per replica: tile counts -> bucket starts
-//   k_bin_scatter     thread per bead: bucket fill (arbitrary order)
-//   k_bin_rank        thread per bucket entry: order := ascending bead index
-//   k_spread_ustar    workgroup per tile: wave d accumulates component d of
-//                     the tile's beads in bucket order into LDS (lanes =
-//                     stencil points, distinct cells, no atomics), then
-//                     u* = u + (mu dt L7 u + f) / rho for the tile's cells
-//   k_fft_x_r2c       rows along x, two real rows per complex transform
-//   k_fft_y           columns along y
-//   k_fft_z_proj      columns along z: forward DFT, thermal forcing in k-space,
-//                     discrete Helmholtz projection, k=0 removal, 1/N, inverse
-//   k_fft_y           columns along y (inverse)
-//   k_fft_x_c2r       rows along x -> u^{n+1}
+//   k_bead_update     16 lanes per bead (4 beads per wave): 4x4x4 Peskin
+//                     gather of u^n for the three staggered components, drag +
+//                     thermal kick, x/v/f update, the bead's spread record,
+//                     insertion into the buckets of the tiles its stencils touch
+//   k_spread          workgroup per tile: bucket sorted by bead index in LDS;
+//                     wave d accumulates component d (lanes = stencil points,
+//                     distinct cells, no atomics) and the tile goes to the
+//                     sparse pool; untouched tiles are marked empty
+//   [k_ustar]         u* = u + (mu dt L7 u + f) / rho (runtime-length FFT path
+//                     only; the compile-time-length X pass computes u* itself)
+//   X, Y, Z+projection, Y, X passes of the 3-D FFT (selm_fft_kernels.hip for
+//                     the compile-time lengths, below for the others)
 //   k_bond_forces     thread per bead: harmonic bonds + external force at
 //                     x^{n+1}; advances each replica's step counter
+#include "selm_fluid.hpp"
 #include "selm_kernels.hpp"
 
 namespace selm {
@@ -41,29 +37,35 @@ __device__ __forceinline__ int axis_tiles(double s, int n, int te, int *out) {
     return cnt;
 }
 
-__device__ __forceinline__ int bead_tiles(const RepDev &R, double s0, double s1, double s2, int *tiles) {
-    int tx[6], ty[6], tz[6];
-    const int cx = axis_tiles(s0, R.n[0], R.te[0], tx);
-    const int cy = axis_tiles(s1, R.n[1], R.te[1], ty);
-    const int cz = axis_tiles(s2, R.n[2], R.te[2], tz);
-    int k = 0;
-    for (int c = 0; c < cz; c++)
-        for (int b = 0; b < cy; b++)
-            for (int a = 0; a < cx; a++) tiles[k++] = (tz[c] * R.tn[1] + ty[b]) * R.tn[0] + tx[a];
-    return k;
+// Peskin factor with the branch decided on the FP64 distance (so the r >= 2
+// quirk and the continuity at r -> 2^- survive) and the arithmetic in T.
+template <typename T>
+__device__ __forceinline__ T phi_t(double r, int f4) {
+    if constexpr (sizeof(T) == 8) {
+        return peskin_phi(r, f4);
+    } else {
+        const float q = (float)r;
+        if (r < 1.0) return (3.0f - 2.0f * q + sqrtf(1.0f + 4.0f * q * (1.0f - q))) * 0.125f;
+        if (r < 2.0) return (5.0f - 2.0f * q - sqrtf(fmaxf(-7.0f + 4.0f * q * (3.0f - q), 0.0f))) * 0.125f;
+        return f4 ? 1.0f : 0.0f;
+    }
 }
 
 // ===========================================================================
 // Beads: gather, drag, thermal kick, update.  Semantics of
 // cuda/SELM_Kernel.cu:302-373 and mpi/SELM_Lagrangian_LAMMPS.cpp:115-180.
 // ===========================================================================
+constexpr int kBeadThreads = 256;
+constexpr int kBeadsPerBlock = kBeadThreads / 16;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ reps, int nrep,
-                                                     const long long *__restrict__ start,
-                                                     long long total) {
-    const int lane = threadIdx.x & 63;
-    const long long g = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (g >= total) return;  // uniform per wave
+__global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__restrict__ reps, int nrep,
+                                                              const long long *__restrict__ start,
+                                                              long long total) {
+    const int sl = threadIdx.x & 15;  // sub-lane: (li, lj) column of the stencil
+    const long long g0 = (long long)blockIdx.x * kBeadsPerBlock + (threadIdx.x >> 4);
+    const bool active = g0 < total;
+    const long long g = active ? g0 : total - 1;  // inactive groups mirror the last bead, store nothing
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
     const int b = (int)(g - start[r]);
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
 #pragma unroll
     for (int d = 0; d < 3; d++) s[d] = (R.x[3 * b + d] - R.lo) / R.dx;
 
-    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
+    const int li = sl & 3, lj = sl >> 2;
     const T *u = (const T *)R.u;
     double gam[3];
 #pragma unroll
@@ -86,31 +88,59 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
         const double ry = s[1] - (d != 1 ? 0.5 : 0.0);
         const double rz = s[2] - (d != 2 ? 0.5 : 0.0);
         const int p0 = (int)(floor(rx) - 1.0), p1 = (int)(floor(ry) - 1.0), p2 = (int)(floor(rz) - 1.0);
-        const int pi = p0 + li, pj = p1 + lj, pk = p2 + lk;
-        const double fz = peskin_phi(fabs(rz - pk), f4);
-        const double fy = peskin_phi(fabs(ry - pj), f4);
-        const double fx = peskin_phi(fabs(rx - pi), f4);
-        const double w = ((1.0 * fz) * fy) * fx;
-        const long long idx = ((long long)wrap_idx(pk, nz) * ny + wrap_idx(pj, ny)) * nx + wrap_idx(pi, nx);
-        const T val = u[d * N + idx] * (T)w;
-        gam[d] = (double)wave_sum<T>(val);
-        if (lj == 0 && lk == 0) rec->w[d][0][li] = (T)fx;
-        if (li == 0 && lk == 0) rec->w[d][1][lj] = (T)fy;
-        if (li == 0 && lj == 0) rec->w[d][2][lk] = (T)fz;
-        if (lane == 0) {
-            rec->p[d][0] = wrap_idx(p0, nx);
-            rec->p[d][1] = wrap_idx(p1, ny);
-            rec->p[d][2] = wrap_idx(p2, nz);
+        const T fx = phi_t<T>(fabs(rx - (p0 + li)), f4);
+        const T fy = phi_t<T>(fabs(ry - (p1 + lj)), f4);
+        const long long col = (long long)wrap_idx(p1 + lj, ny) * nx + wrap_idx(p0 + li, nx);
+        const T *ud = u + d * N;
+        T fz[4], uv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            fz[k] = phi_t<T>(fabs(rz - (p2 + k)), f4);
+            uv[k] = ud[(long long)wrap_idx(p2 + k, nz) * ny * nx + col];
+        }
+        T acc = (T)0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc += uv[k] * ((((T)1 * fz[k]) * fy) * fx);
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) acc += __shfl_xor(acc, m, 64);
+        gam[d] = (double)acc;
+        if (active) {
+            if (lj == 0) rec->w[d][0][li] = fx;
+            if (li == 0) rec->w[d][1][lj] = fy;
+            if (sl < 4) rec->w[d][2][sl] = sl == 0 ? fz[0] : (sl == 1 ? fz[1] : (sl == 2 ? fz[2] : fz[3]));
+            if (sl == 0) {
+                rec->p[d][0] = wrap_idx(p0, nx);
+                rec->p[d][1] = wrap_idx(p1, ny);
+                rec->p[d][2] = wrap_idx(p2, nz);
+            }
         }
     }
+    if (!active) return;
 
-    if (lane == 0) {
-        int tiles[kMaxTilesPerBead];
-        const int nt = bead_tiles(R, s[0], s[1], s[2], tiles);
-        for (int q = 0; q < nt; q++) atomicAdd(&R.tcount[tiles[q]], 1);
+    const uint32_t step = (uint32_t)(R.ts[0] + 1);
+    {
+        // insert into the bucket of every tile the stencils touch; the bead's
+        // 16 lanes take one (tile) insertion each
+        int tx[6], ty[6], tz[6];
+        const int cx = axis_tiles(s[0], nx, R.te[0], tx);
+        const int cy = axis_tiles(s[1], ny, R.te[1], ty);
+        const int cz = axis_tiles(s[2], nz, R.te[2], tz);
+        const int ncomb = cx * cy * cz;
+        for (int q = sl; q < ncomb; q += 16) {
+            const int a = q % cx, qb = (q / cx) % cy, c = q / (cx * cy);
+            const int t = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
+            const int slot = atomicAdd(&R.bcnt[t], 1);
+            if (slot < kBucketCap) {
+                R.bucket[t * kBucketCap + slot] = b;
+            } else {
+                const int o = atomicAdd(&R.ctr[step & 1], 1);
+                R.ovf[2 * o] = t;
+                R.ovf[2 * o + 1] = b;
+            }
+        }
     }
-    if (lane < 3) {
-        const int d = lane;
+    if (sl < 3) {
+        const int d = sl;
         const double gd = d == 0 ? gam[0] : (d == 1 ? gam[1] : gam[2]);
         const double sd = d == 0 ? s[0] : (d == 1 ? s[1] : s[2]);
         const int i = 3 * b + d;
@@ -118,7 +148,6 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
         const double vi = R.v[i];
         double tf = -R.ups * (vi - gd) * dt;
         if (R.noise) {
-            const uint32_t step = (uint32_t)(R.ts[0] + 1);
             double z0, z1;
             normal_pair_f64(R.seed, kStreamBead, step, (uint32_t)i, 0u, z0, z1);
             tf += z0 * R.bead_coef;
@@ -138,91 +167,11 @@ __global__ __launch_bounds__(256) void k_bead_update(const RepDev *__restrict__ 
 }
 
 // ===========================================================================
-// Binning: counting sort of (bead, tile) entries by tile, deterministic order.
-// ===========================================================================
-__global__ __launch_bounds__(1024) void k_bin_scan(const RepDev *__restrict__ reps, int nrep) {
-    __shared__ int wsum[16];
-    const RepDev &R = reps[blockIdx.x];
-    const int nt = R.ntiles;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int per = (nt + 1023) >> 10;
-    const int beg = min(tid * per, nt), end = min(beg + per, nt);
-    int local = 0;
-    for (int t = beg; t < end; t++) local += R.tcount[t];
-    // block-wide exclusive scan of `local`
-    int inc = local;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += y;
-    }
-    if (lane == 63) wsum[wv] = inc;
-    __syncthreads();
-    if (wv == 0) {
-        int v = lane < 16 ? wsum[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            int y = __shfl_up(v, o, 64);
-            if (lane >= o) v += y;
-        }
-        if (lane < 16) wsum[lane] = v;
-    }
-    __syncthreads();
-    int run = inc - local + (wv > 0 ? wsum[wv - 1] : 0);
-    for (int t = beg; t < end; t++) {
-        const int c = R.tcount[t];
-        R.tstart[t] = run;
-        run += c;
-        R.tcount[t] = 0;
-        R.tcursor[t] = 0;
-    }
-    if (tid == 1023) R.tstart[nt] = wsum[15];
-}
-
-__global__ __launch_bounds__(256) void k_bin_scatter(const RepDev *__restrict__ reps, int nrep,
-                                                     const long long *__restrict__ start,
-                                                     long long total) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const int b = (int)(g - start[r]);
-    int tiles[kMaxTilesPerBead];
-    const int nt = bead_tiles(R, R.sg[3 * b], R.sg[3 * b + 1], R.sg[3 * b + 2], tiles);
-    for (int q = 0; q < nt; q++) {
-        const int t = tiles[q];
-        const int pos = R.tstart[t] + atomicAdd(&R.tcursor[t], 1);
-        R.permu[pos] = b;
-        R.etile[pos] = t;
-    }
-}
-
-// thread per entry slot (27 per bead); slots beyond the filled total exit
-__global__ __launch_bounds__(256) void k_bin_rank(const RepDev *__restrict__ reps, int nrep,
-                                                  const long long *__restrict__ start,
-                                                  long long total) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const int q = (int)(g - start[r]);
-    if (q >= R.tstart[R.ntiles]) return;
-    const int t = R.etile[q];
-    const int b = R.permu[q];
-    const int st = R.tstart[t], en = R.tstart[t + 1];
-    int rank = 0;
-    for (int e = st; e < en; e++) rank += (R.permu[e] < b) ? 1 : 0;
-    R.perm[st + rank] = b;
-}
-
-// ===========================================================================
-// Spread (Lambda) + viscous predictor, workgroup per tile.
-// Spread semantics of cuda/SELM_Kernel.cu:424-470: each bead adds -t_f * w
-// to the 4x4x4 stencil of each staggered component, w = ((1*phi_z)*phi_y)*phi_x.
-// Predictor semantics of cuda/SELM_Kernel.cu:155-195:
-// u* = u + (mu * L7(u) * dt + f) / rho, L7 the periodic 7-point Laplacian / dx^2.
-// Each cell's spread sum runs over the tile's bucket in ascending bead index,
-// so the result does not depend on scheduling or on other replicas.
+// Spread (Lambda), workgroup per tile.  Semantics of cuda/SELM_Kernel.cu:424-470:
+// each bead adds -t_f * w to the 4x4x4 stencil of each staggered component,
+// w = ((1*phi_z)*phi_y)*phi_x.  Each cell's sum runs over the tile's beads in
+// ascending bead index, so the result does not depend on scheduling or on the
+// other replicas of a batch.
 // ===========================================================================
 template <typename T>
 struct Acc;
@@ -243,17 +192,23 @@ struct Acc<float> {  // Neumaier-compensated FP32 sum
 };
 
 constexpr int kTileCells = 512;  // tile edge <= 8
-constexpr int kSpreadChunk = 64;
+constexpr int kSpreadChunk = 32;
 constexpr int kSpreadThreads = 256;
+constexpr int kListCap = 2048;
 
 template <typename T>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread_ustar(const RepDev *__restrict__ reps,
-                                                                 int nrep,
-                                                                 const long long *__restrict__ start,
-                                                                 long long total) {
-    __shared__ T acc_s[3][kTileCells];
-    __shared__ T acc_c[3][kTileCells];
+__global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps, int nrep,
+                                                           const long long *__restrict__ start,
+                                                           long long total) {
+    constexpr int ACCW = sizeof(T) == 4 ? 2 * 3 * kTileCells : 3 * kTileCells;  // sum (+ compensation)
+    static_assert(ACCW * sizeof(T) >= kListCap * sizeof(int), "list shares the accumulator space");
+    __shared__ int sorted[kListCap];
+    __shared__ T accbuf[ACCW];  // holds the unsorted list first, then the accumulators
+    __shared__ int nlist;
     __shared__ SpreadRec<T> rec[kSpreadChunk];
+    int *list = (int *)accbuf;
+    T *acc_s = accbuf;
+    T *acc_c = accbuf + (sizeof(T) == 4 ? 3 * kTileCells : 0);
 
     const long long g = blockIdx.x;
     if (g >= total) return;
@@ -261,26 +216,54 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_ustar(const RepDev *_
     const RepDev &R = reps[r];
     const int t = (int)(g - start[r]);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t step = (uint32_t)(R.ts[0] + 1);
+    const int par = step & 1;
+    if (t == 0 && tid == 0) R.ctr[par ^ 1] = 0;  // next step's overflow count (last used a step ago)
+    const int cnt = R.bcnt[t];
+    if (cnt == 0) return;  // no bead touches this tile: its stamp stays stale
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
     const int ox = tx * R.te[0], oy = ty * R.te[1], oz = tz * R.te[2];
     const int ex = min(R.te[0], nx - ox), ey = min(R.te[1], ny - oy), ez = min(R.te[2], nz - oz);
-    const int ncell = ex * ey * ez;
 
-    for (int c = tid; c < 3 * kTileCells; c += kSpreadThreads) {
-        (&acc_s[0][0])[c] = (T)0;
-        (&acc_c[0][0])[c] = (T)0;
+    const int nb = min(cnt, kBucketCap);
+    if (tid == 0) nlist = nb;
+    for (int i = tid; i < nb; i += kSpreadThreads) list[i] = R.bucket[t * kBucketCap + i];
+    __syncthreads();
+    if (cnt > kBucketCap) {
+        const int novf = R.ctr[par];
+        for (int o = tid; o < novf; o += kSpreadThreads)
+            if (R.ovf[2 * o] == t) {
+                const int p = atomicAdd(&nlist, 1);
+                if (p < kListCap) list[p] = R.ovf[2 * o + 1];
+            }
+        __syncthreads();
     }
-    const int st = R.tstart[t], en = R.tstart[t + 1];
+    const int n = min(nlist, kListCap);
+    if (tid == 0) {
+        if (nlist > kListCap) atomicOr(&R.ctr[4], kStatusListOverflow);
+        R.bcnt[t] = 0;  // ready for the next step's insertions
+    }
+    // order the tile's beads by index (indices are unique within a tile)
+    for (int e = tid; e < n; e += kSpreadThreads) {
+        const int bi = list[e];
+        int rank = 0;
+        for (int q = 0; q < n; q++) rank += (list[q] < bi) ? 1 : 0;
+        sorted[rank] = bi;
+    }
+    __syncthreads();
+    for (int c = tid; c < ACCW; c += kSpreadThreads) accbuf[c] = (T)0;
+
     const SpreadRec<T> *brec = (const SpreadRec<T> *)R.brec;
     constexpr int W = sizeof(SpreadRec<T>) / 4;
     const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
-    for (int base = st; base < en; base += kSpreadChunk) {
-        const int nq = min(kSpreadChunk, en - base);
-        __syncthreads();  // previous chunk consumed / accumulators cleared
+    const int te0 = R.te[0], te1 = R.te[1];
+    for (int base = 0; base < n; base += kSpreadChunk) {
+        const int nq = min(kSpreadChunk, n - base);
+        __syncthreads();  // accumulators cleared / previous chunk consumed
         for (int idx = tid; idx < nq * W; idx += kSpreadThreads) {
             const int k = idx / W, wd = idx - k * W;
-            ((uint32_t *)&rec[k])[wd] = ((const uint32_t *)&brec[R.perm[base + k]])[wd];
+            ((uint32_t *)&rec[k])[wd] = ((const uint32_t *)&brec[sorted[base + k]])[wd];
         }
         __syncthreads();
         if (wv < 3) {
@@ -298,38 +281,40 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_ustar(const RepDev *_
                 cz -= oz;
                 if ((unsigned)cx < (unsigned)ex && (unsigned)cy < (unsigned)ey && (unsigned)cz < (unsigned)ez) {
                     const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
-                    const int cell = (cz * ey + cy) * ex + cx;
-                    Acc<T>::add(&acc_s[d][cell], &acc_c[d][cell], B.f[d] * w);
+                    const int cell = d * kTileCells + (cz * te1 + cy) * te0 + cx;
+                    Acc<T>::add(&acc_s[cell], &acc_c[sizeof(T) == 4 ? cell : 0], B.f[d] * w);
                 }
             }
         }
     }
     __syncthreads();
-
-    const long long N = R.N, Nyx = (long long)nx * ny;
-    const T dxsq = (T)(R.dx * R.dx);
-    const T mu = (T)R.mu, dt = (T)R.dt, rho = (T)R.rho;
-    const T *u = (const T *)R.u;
-    T *out = (T *)R.fsp;
-    for (int c = tid; c < ncell; c += kSpreadThreads) {
-        const int cx = c % ex, cy = (c / ex) % ey, cz = c / (ex * ey);
-        const int i = ox + cx, j = oy + cy, k = oz + cz;
-        const long long n = (long long)k * Nyx + (long long)j * nx + i;
-        const long long i1 = (i != nx - 1) ? 1 : -i;
-        const long long j1 = (j != ny - 1) ? nx : -(long long)j * nx;
-        const long long k1 = (k != nz - 1) ? Nyx : -(long long)k * Nyx;
-        const long long i0 = (i != 0) ? -1 : nx - 1;
-        const long long j0 = (j != 0) ? -(long long)nx : (long long)(ny - 1) * nx;
-        const long long k0 = (k != 0) ? -Nyx : (long long)(nz - 1) * Nyx;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const T *ud = u + d * N;
-            const T uc = ud[n];
-            const T lap = (ud[n + i1] + ud[n + i0] + ud[n + j1] + ud[n + j0] + ud[n + k1] + ud[n + k0] - (T)6 * uc) / dxsq;
-            const T f = Acc<T>::value(&acc_s[d][c], &acc_c[d][c]);
-            out[d * N + n] = uc + (mu * lap * dt + f) / rho;
-        }
+    const int tcells = te0 * te1 * R.te[2];
+    T *pool = (T *)R.pool + (long long)t * 3 * tcells;
+    for (int c = tid; c < 3 * tcells; c += kSpreadThreads) {
+        const int d = c / tcells, cell = d * kTileCells + (c - d * tcells);
+        pool[c] = Acc<T>::value(&acc_s[cell], &acc_c[sizeof(T) == 4 ? cell : 0]);
     }
+    if (tid == 0) R.tstamp[t] = (int)step;
+}
+
+// u* over the whole grid into fsp (runtime-length FFT path only)
+template <typename T>
+__global__ __launch_bounds__(256) void k_ustar(const RepDev *__restrict__ reps, int nrep,
+                                               const long long *__restrict__ start, long long total) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int r = find_rep(start, nrep, g);
+    const RepDev &R = reps[r];
+    const long long n = g - start[r];
+    const int nx = R.n[0], ny = R.n[1];
+    const long long Nyx = (long long)nx * ny;
+    const int z = (int)(n / Nyx), y = (int)((n % Nyx) / nx), x = (int)(n % nx);
+    int tyz, lyz;
+    spread_row(R, y, z, tyz, lyz);
+    const int step = (int)(R.ts[0] + 1);
+    T *out = (T *)R.fsp;
+#pragma unroll
+    for (int d = 0; d < 3; d++) out[d * R.N + n] = ustar_at<T>(R, d, x, y, z, tyz, lyz, step);
 }
 
 // ===========================================================================
@@ -341,8 +326,12 @@ __device__ __forceinline__ void load_twiddles(cplx<T> *dst, const cplx<T> *src, 
     for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = src[m];
 }
 
-// Forward r2c along x: rows rho = (d*nz + z)*ny + y of length nx; two real
-// rows form one complex sequence z = a + i b, separated after the DFT.
+// ===========================================================================
+// Runtime-length FFT passes (any edge that is a product of 2, 3 and 5); the
+// compile-time lengths use selm_fft_kernels.hip.
+// ===========================================================================
+// Forward r2c along x of u* (fsp): rows rho = (d*nz + z)*ny + y of length nx;
+// two real rows form one complex sequence z = a + i b, separated after the DFT.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_x_r2c(const RepDev *__restrict__ reps, int nrep,
                                                    const long long *__restrict__ start,
@@ -652,25 +641,15 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
     switch (stage) {
         case kStageBead:
             if (L.tot_bead > 0)
-                hipLaunchKernelGGL(k_bead_update<T>, dim3(nblk(L.tot_bead, 4)), dim3(256), 0, st, reps, nrep,
-                                   L.st_bead, L.tot_bead);
-            break;
-        case kStageBin:
-            hipLaunchKernelGGL(k_bin_scan, dim3(nrep), dim3(1024), 0, st, reps, nrep);
-            if (L.tot_bead > 0) {
-                hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(L.tot_bead, 256)), dim3(256), 0, st, reps, nrep,
-                                   L.st_bead, L.tot_bead);
-                hipLaunchKernelGGL(k_bin_rank, dim3(nblk(L.tot_ent, 256)), dim3(256), 0, st, reps, nrep,
-                                   L.st_ent, L.tot_ent);
-            }
+                hipLaunchKernelGGL(k_bead_update<T>, dim3(nblk(L.tot_bead, kBeadsPerBlock)), dim3(kBeadThreads), 0,
+                                   st, reps, nrep, L.st_bead, L.tot_bead);
             break;
         case kStageSpread:
-            hipLaunchKernelGGL(k_spread_ustar<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads), 0, st, reps,
-                               nrep, L.st_tile, L.tot_tile);
-            break;
-        case kStageBonds:
-            hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
-                               reps, nrep, L.st_bead, L.tot_bead, 1);
+            hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads), 0, st, reps, nrep,
+                               L.st_tile, L.tot_tile);
+            if (!L.fast)
+                hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep, L.st_pts,
+                                   L.tot_pts);
             break;
         case kStageFftX:
         case kStageFftY:
@@ -679,6 +658,10 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
         case kStageFftXi:
             if (L.fast) launch_fft_fast(L, stage, st);
             else launch_fft_generic<T>(L, stage, st);
+            break;
+        case kStageBonds:
+            hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
+                               reps, nrep, L.st_bead, L.tot_bead, 1);
             break;
         default: break;
     }

@@ -9,9 +9,9 @@ namespace selm {
 
 enum Stage {
     kStageBead = 0,
-    kStageBin,
-    kStageSpread,  // spread + viscous predictor u*
-    kStageFftX,
+    kStageSpread,
+    kStageFftX,  // + viscous predictor u* on the compile-time-length path
+
     kStageFftY,
     kStageFftZ,
     kStageFftYi,
@@ -25,8 +25,8 @@ struct StepLaunch {
     const RepDev *reps;   // device array [nrep]
     int nrep;
     bool fp64;
-    const long long *st_bead, *st_tile, *st_ent, *st_fx, *st_fy, *st_fz;  // device [nrep+1]
-    long long tot_bead, tot_tile, tot_ent, tot_fx, tot_fy, tot_fz;
+    const long long *st_bead, *st_tile, *st_pts, *st_fx, *st_fy, *st_fz;  // device [nrep+1]
+    long long tot_bead, tot_tile, tot_pts, tot_fx, tot_fy, tot_fz;
     size_t lds_fx, lds_fy, lds_fz;   // dynamic LDS bytes per pass
     int twmax;                       // LDS twiddle slots (elements) preceding the data
     bool fast;                       // compile-time-length FFT kernels (selm_fft_kernels.hip)
