@@ -65,11 +65,12 @@ def stage_bytes(w, s):
     M = w.M
     rec = 208 if s == 4 else 368  # spread record written per bead
     return {
+        "thermal_noise": 3 * Nc * 2 * s,
         "bead_update": M * (192 * s + 24 + 152 + rec),
         "spread": M * (24 + 3 * s) + 192 * M * s,
         "ustar_fft_x": 3 * N * s + 3 * Nc * 2 * s,
         "fft_y": 2 * 3 * Nc * 2 * s,
-        "fft_z_projection": 2 * 3 * Nc * 2 * s,
+        "fft_z_projection": 3 * 3 * Nc * 2 * s,
         "ifft_y": 2 * 3 * Nc * 2 * s,
         "ifft_x": 3 * Nc * 2 * s + 3 * N * s,
         "bond_forces": M * (48 + 24 * 2 + 24),

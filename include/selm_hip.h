@@ -40,7 +40,7 @@ extern "C" {
 #define SELM_PREC_FP32_BULK 1
 
 /* per-stage indices for selm_stage_times */
-#define SELM_NUM_STAGES 8
+#define SELM_NUM_STAGES 9
 
 typedef struct selm_ctx selm_ctx;
 typedef struct selm_pool selm_pool;
@@ -138,8 +138,10 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
 
 /* Run nsteps steps stage by stage with HIP events around each stage on the
  * ctx stream; ms[s] receives the summed milliseconds of stage s
- * (0 bead update + binning, 1 spread, 2 viscous u* + FFT x, 3 FFT y,
- *  4 FFT z + projection, 5 inverse FFT y, 6 inverse FFT x, 7 bond forces). */
+ * (0 thermal forcing, 1 bead update + binning, 2 spread, 3 viscous u* + FFT x,
+ *  4 FFT y, 5 FFT z + projection, 6 inverse FFT y, 7 inverse FFT x,
+ *  8 bond forces).  In selm_step the thermal forcing runs concurrently with
+ *  stages 1-4; here every stage runs alone. */
 int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
 
 /* Sum over the grid of u* for each component before the k=0 mode is

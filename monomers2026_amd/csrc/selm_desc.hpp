@@ -20,7 +20,8 @@ struct RepDev {
     int n[3];             // nx, ny, nz
     int xc;               // nx/2 + 1
     long long N, Nc;      // nx*ny*nz, xc*ny*nz
-    void *u, *fsp, *spec;
+    void *u, *fsp, *spec, *spec2;   // spec2: second spectrum layout of the transposing FFT
+    void *nbuf;           // thermal forcing of this step in k-space, [3][xc][ny][nz] complex T
     void *pool;           // spread impulse per tile: [tile][3][te0*te1*te2] of T, valid when
     int *tstamp;          // tstamp[tile] == the current step (a bead touched the tile)
     const void *tw[3];    // cplx<T>[n_a]: e^{-2 pi i m / n_a}

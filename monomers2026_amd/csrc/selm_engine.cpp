@@ -113,7 +113,7 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     size_t s = fp64 ? 8 : 4;
     size_t b = 0;
     b += 2 * al(3 * (size_t)G.N * s);     // u, fsp
-    b += al(3 * (size_t)G.Nc * 2 * s);    // spec
+    b += 3 * al(3 * (size_t)G.Nc * 2 * s);    // spec, spec2, nbuf
     for (int a = 0; a < 3; a++) b += al((size_t)G.n[a] * 2 * s) + al(4 * (size_t)G.n[a] * s);
     b += al(8) + al(3 * 8);               // ts, diag
     const size_t tcells = (size_t)G.te[0] * G.te[1] * G.te[2];
@@ -164,6 +164,8 @@ static void carve_grid(Replica &R, char *base) {
     R.D.u = take(3 * (size_t)G.N * s);
     R.D.fsp = take(3 * (size_t)G.N * s);
     R.D.spec = take(3 * (size_t)G.Nc * 2 * s);
+    R.D.spec2 = take(3 * (size_t)G.Nc * 2 * s);
+    R.D.nbuf = take(3 * (size_t)G.Nc * 2 * s);
     for (int a = 0; a < 3; a++) {
         R.D.tw[a] = take((size_t)G.n[a] * 2 * s);
         R.D.ktab[a] = take(4 * (size_t)G.n[a] * s);
@@ -413,6 +415,8 @@ struct Batch {
     bool use_graph = true;
     bool force_generic_fft = false;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<Replica *> reps;
     RepDev *d_reps = nullptr;
     int cap_reps = 0;
@@ -440,17 +444,19 @@ struct Batch {
             HIPCHK(hipMalloc(&d_reps, sizeof(RepDev) * cap));
             cap_reps = cap;
         }
-        if (6 * (nrep + 1) > cap_tabs) {
+        if (7 * (nrep + 1) > cap_tabs) {
             if (d_tabs) (void)hipFree(d_tabs);
             d_tabs = nullptr;
-            int cap = std::max(6 * (nrep + 1), 2 * cap_tabs);
+            int cap = std::max(7 * (nrep + 1), 2 * cap_tabs);
             HIPCHK(hipMalloc(&d_tabs, sizeof(long long) * cap));
             cap_tabs = cap;
         }
         std::vector<RepDev> h(nrep);
-        std::vector<long long> tabs(6 * (nrep + 1), 0);
+        std::vector<long long> tabs(7 * (nrep + 1), 0);
         long long *sb = tabs.data(), *stl = sb + (nrep + 1), *sen = stl + (nrep + 1);
         long long *sfx = sen + (nrep + 1), *sfy = sfx + (nrep + 1), *sfz = sfy + (nrep + 1);
+        long long *smo = sfz + (nrep + 1);
+        bool any_noise = false;
         const size_t csz = fp64 ? 16 : 8;
         int twmax = 0;
         for (int r = 0; r < nrep; r++)
@@ -466,27 +472,30 @@ struct Batch {
         }
         for (int r = 0; r < nrep; r++) {
             const Replica &R = *reps[r];
-            Geometry G = R.G;
-            if (fast) fft_fast_blocks(fp64, &G.bx, &G.by, &G.bz);
+            const Geometry &G = R.G;
             h[r] = R.D;
             h[r].M = R.M;
-            h[r].bx = G.bx;
-            h[r].by = G.by;
-            h[r].bz = G.bz;
             sb[r + 1] = sb[r] + R.M;
             stl[r + 1] = stl[r] + G.ntiles;
             sen[r + 1] = sen[r] + G.N;
-            long long rows = 3LL * G.n[1] * G.n[2];
-            sfx[r + 1] = sfx[r] + (rows + G.bx - 1) / G.bx;
-            sfy[r + 1] = sfy[r] + 3LL * G.n[2] * ((G.xc + G.by - 1) / G.by);
-            sfz[r + 1] = sfz[r] + ((long long)G.n[1] * G.xc + G.bz - 1) / G.bz;
+            smo[r + 1] = smo[r] + G.Nc;
+            any_noise = any_noise || R.D.noise;
             if (fast) {
+                long long ix, iy, iz;
+                fft_fast_items(fp64, G.n, &ix, &iy, &iz);
+                sfx[r + 1] = sfx[r] + ix;
+                sfy[r + 1] = sfy[r] + iy;
+                sfz[r + 1] = sfz[r] + iz;
                 size_t a, b, c;
                 fft_fast_lds(fp64, G.n, twmax, &a, &b, &c);
                 lfx = std::max(lfx, a);
                 lfy = std::max(lfy, b);
                 lfz = std::max(lfz, c);
             } else {
+                long long rows = 3LL * G.n[1] * G.n[2];
+                sfx[r + 1] = sfx[r] + (rows + G.bx - 1) / G.bx;
+                sfy[r + 1] = sfy[r] + 3LL * G.n[2] * ((G.xc + G.by - 1) / G.by);
+                sfz[r + 1] = sfz[r] + ((long long)G.n[1] * G.xc + G.bz - 1) / G.bz;
                 lfx = std::max(lfx, lds_x(G, twmax, csz));
                 lfy = std::max(lfy, lds_y(G, twmax, csz));
                 lfz = std::max(lfz, lds_z(G, twmax, csz));
@@ -511,18 +520,32 @@ struct Batch {
         L.st_fx = d_tabs + 3 * (nrep + 1);
         L.st_fy = d_tabs + 4 * (nrep + 1);
         L.st_fz = d_tabs + 5 * (nrep + 1);
+        L.st_modes = d_tabs + 6 * (nrep + 1);
         L.tot_bead = sb[nrep];
         L.tot_tile = stl[nrep];
         L.tot_pts = sen[nrep];
         L.tot_fx = sfx[nrep];
         L.tot_fy = sfy[nrep];
         L.tot_fz = sfz[nrep];
+        L.tot_modes = (fast && any_noise) ? smo[nrep] : 0;
+        L.st2 = stream2;
+        L.ev_fork = ev_fork;
+        L.ev_join = ev_join;
         L.lds_fx = lfx;
         L.lds_fy = lfy;
         L.lds_fz = lfz;
         L.twmax = twmax;
         L.fast = fast;
         dirty = false;
+        return SELM_OK;
+    }
+
+    // both streams at the default priority
+    int init_streams(std::string &err) {
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         return SELM_OK;
     }
 
@@ -560,7 +583,12 @@ struct Batch {
         d_reps = nullptr;
         d_tabs = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
+        if (stream2) (void)hipStreamDestroy(stream2);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         stream = nullptr;
+        stream2 = nullptr;
+        ev_fork = ev_join = nullptr;
     }
 };
 
@@ -649,13 +677,13 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     c->batch.fp64 = fp64;
     // SELM_FFT_GENERIC=1 selects the runtime-length FFT kernels for every size
     c->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&c->batch.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
+    if (c->batch.init_streams(err) != SELM_OK) {
+        c->batch.release();
         delete c;
         return SELM_ERR_HIP;
     }
     size_t gb = grid_bytes(c->rep.G, fp64);
-    e = hipMalloc(&c->grid_mem, gb);
+    hipError_t e = hipMalloc(&c->grid_mem, gb);
     if (e != hipSuccess) {
         c->batch.release();
         delete c;
@@ -948,7 +976,8 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->batch.fp64 = pp->precision == SELM_PREC_FP64;
     P->batch.use_graph = pp->use_graph != 0;
     P->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
-    if (hipStreamCreateWithFlags(&P->batch.stream, hipStreamNonBlocking) != hipSuccess) {
+    if (P->batch.init_streams(P->err) != SELM_OK) {
+        P->batch.release();
         delete P;
         return SELM_ERR_HIP;
     }

@@ -1,15 +1,22 @@
-// selm_fft_kernels.hip -- the 3-D FFT passes of the step on the specialised
-// (compile-time length) path.  Each kernel switches on the replica's edge
+// selm_fft_kernels.hip -- the 3-D FFT of the step on the compile-time-length
+// path, as a transposing pipeline: every pass reads whole contiguous rows of
+// its transform axis and writes its output transposed, so that the next
+// pass's axis is contiguous in memory.  Every global access is a contiguous
+// run of >= 256 bytes (full rows, or row-blocks of RB consecutive elements).
+//
+//   pass   reads                      transform               writes
+//   Xf     u^n rows (x fastest), u*   r2c along x             B[d][z][kx][y]
+//   Yf     B rows (y fastest)         DFT along y             C[d][kx][ky][z]
+//   Z      C rows (z fastest)         DFT along z, thermal    C in place
+//                                     forcing, projection,
+//                                     k=0 removal, 1/N, iDFT
+//   Yi     C (RB z per row-block)     iDFT along y            B rows
+//   Xi     B (RB y per row-block)     c2r along x             u^{n+1} rows
+//
+// B lives in spec, C in spec2.  Each kernel switches on the replica's edge
 // length per workgroup, so one launch still covers a heterogeneous batch.
-//
-//   k_fx_r2c    rows along x of u* (evaluated on load from u^n and the sparse
-//               spread): two real rows packed as one complex sequence
-//   k_fy        columns along y (forward or inverse), in place on spec
-//   k_fz_proj   columns along z: forward DFT, thermal forcing in k-space,
-//               Helmholtz projection, k=0 removal, 1/N, inverse DFT
-//   k_fx_c2r    rows along x back to real u^{n+1}
-//
-// Spectrum layout: spec[d][z][y][xc], xc = nx/2+1, unnormalised DFT.
+#include <algorithm>
+
 #include "selm_fft.hpp"
 #include "selm_fluid.hpp"
 #include "selm_kernels.hpp"
@@ -20,13 +27,15 @@ template <typename T>
 struct FastCfg;
 template <>
 struct FastCfg<float> {
-    static constexpr int BX = 32, BY = 16, BZ = 8;           // rows / columns per workgroup
-    static constexpr int NTX = 256, NTY = 256, NTZ = 256;     // threads per workgroup
+    static constexpr int RB = 32;  // rows per block in Xf/Xi, z per block in Yf/Yi (256 B runs)
+    static constexpr int Q = 16;   // (kx, ky) columns per Z workgroup, times 3 components
+    static constexpr int NT = 256, NTZ = 512;
 };
 template <>
 struct FastCfg<double> {
-    static constexpr int BX = 16, BY = 8, BZ = 8;
-    static constexpr int NTX = 256, NTY = 256, NTZ = 256;
+    static constexpr int RB = 16;
+    static constexpr int Q = 8;
+    static constexpr int NT = 256, NTZ = 512;
 };
 
 template <typename T>
@@ -35,24 +44,29 @@ __device__ __forceinline__ void tw_to_lds(cplx<T> *dst, const void *src, int n) 
     for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = s[m];
 }
 
-// ------------------------------------------------------------------ X forward
+template <typename T>
+__device__ __forceinline__ cplx<T> czero() {
+    return mk<T>((T)0, (T)0);
+}
+
+// ------------------------------------------------------------------ Xf
+// Workgroup: component d, plane z, rows y0 .. y0+RB-1 of u*; rows are paired
+// into complex sequences (even row = real part, odd row = imaginary part).
 template <typename T, int N>
-__device__ __forceinline__ void fx_r2c_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
-    constexpr int BX = FastCfg<T>::BX, P = BX / 2, S = P + 1, NT = FastCfg<T>::NTX;
-    constexpr int XC = N / 2 + 1;
+__device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+    constexpr int RB = FastCfg<T>::RB, P = RB / 2, S = P + 1, NT = FastCfg<T>::NT, XC = N / 2 + 1;
     const int ny = R.n[1], nz = R.n[2];
-    const long long rows = 3LL * ny * nz;
-    const long long row0 = item * BX;
+    const int nyb = (ny + RB - 1) / RB;
+    const int yb = (int)(item % nyb);
+    const long long dz = item / nyb;
+    const int z = (int)(dz % nz), d = (int)(dz / nz);
+    const int y0 = yb * RB;
     const int step = (int)(R.ts[0] + 1);
-    // rows (d, z, y) of u*, evaluated on load from u^n and the sparse spread
-    for (int idx = threadIdx.x; idx < BX * N; idx += NT) {
+    for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
         const int rl = idx / N, e = idx - rl * N;
-        const long long row = row0 + rl;
+        const int y = y0 + rl;
         T v = (T)0;
-        if (row < rows) {
-            const int y = (int)(row % ny);
-            const long long dzr = row / ny;
-            const int z = (int)(dzr % nz), d = (int)(dzr / nz);
+        if (y < ny) {
             int tyz, lyz;
             spread_row(R, y, z, tyz, lyz);
             v = ustar_at<T>(R, d, e, y, z, tyz, lyz, step);
@@ -63,37 +77,176 @@ __device__ __forceinline__ void fx_r2c_body(const RepDev &R, long long item, cpl
     auto nold = [](int, int) { return cplx<T>{}; };
     auto nost = [](int, int, cplx<T>) {};
     fft_run<T, N, S, NT, false, false>(a, tw, -1, nold, nost);
-    cplx<T> *out = (cplx<T> *)R.spec;
-    for (int idx = threadIdx.x; idx < BX * XC; idx += NT) {
-        const int rl = idx / XC, k = idx - rl * XC;
-        const long long row = row0 + rl;
-        if (row >= rows) continue;
+    // B[d][z][kx][y]: RB consecutive y per kx
+    cplx<T> *out = (cplx<T> *)R.spec + ((long long)d * nz + z) * XC * ny;
+    for (int idx = threadIdx.x; idx < XC * RB; idx += NT) {
+        const int k = idx / RB, rl = idx - k * RB;
+        const int y = y0 + rl;
+        if (y >= ny) continue;
         const int p = rl >> 1;
         const cplx<T> zk = a[k * S + p];
         const cplx<T> zm = a[((N - k) % N) * S + p];
         cplx<T> o;
         if ((rl & 1) == 0) o = mk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
         else o = mk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
-        out[row * XC + k] = o;
+        out[(long long)k * ny + y] = o;
     }
 }
 
-// ------------------------------------------------------------------ X inverse
+// ------------------------------------------------------------------ Yf / Yi
+// Workgroup: component d, x-frequency kx, planes z0 .. z0+RB-1.
+// Forward: B rows (y contiguous) -> DFT_y -> C[d][kx][ky][z0..] (RB z contiguous).
+// Inverse: C -> iDFT_y -> B rows.
+template <typename T, int N, bool FWD>
+__device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+    constexpr int RB = FastCfg<T>::RB, S = RB + 1, NT = FastCfg<T>::NT;
+    const int nz = R.n[2], xc = R.xc;
+    const int nzb = (nz + RB - 1) / RB;
+    const int zb = (int)(item % nzb);
+    const long long dk = item / nzb;
+    const int kx = (int)(dk % xc), d = (int)(dk / xc);
+    const int z0 = zb * RB;
+    const int zv = min(RB, nz - z0);
+    cplx<T> *B = (cplx<T> *)R.spec + (long long)d * nz * xc * N;             // [z][kx][y]
+    cplx<T> *Cc = (cplx<T> *)R.spec2 + ((long long)d * xc + kx) * N * nz;  // [ky][z]
+    if (FWD) {
+        for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
+            const int q = idx / N, y = idx - q * N;
+            a[y * S + q] = q < zv ? B[((long long)(z0 + q) * xc + kx) * N + y] : czero<T>();
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < N * RB; idx += NT) {
+            const int ky = idx / RB, q = idx - ky * RB;
+            a[ky * S + q] = q < zv ? Cc[(long long)ky * nz + z0 + q] : czero<T>();
+        }
+    }
+    __syncthreads();
+    auto nold = [](int, int) { return cplx<T>{}; };
+    auto nost = [](int, int, cplx<T>) {};
+    fft_run<T, N, S, NT, false, false>(a, tw, FWD ? -1 : +1, nold, nost);
+    if (FWD) {
+        for (int idx = threadIdx.x; idx < N * RB; idx += NT) {
+            const int ky = idx / RB, q = idx - ky * RB;
+            if (q < zv) Cc[(long long)ky * nz + z0 + q] = a[ky * S + q];
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
+            const int q = idx / N, y = idx - q * N;
+            if (q < zv) B[((long long)(z0 + q) * xc + kx) * N + y] = a[y * S + q];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Z + projection
+// Workgroup: x-frequency kx, y-frequencies ky0 .. ky0+Q-1, all three
+// components; rows of C are contiguous along z.
+//
+// Thermal forcing (cuda/SELM_Kernel.cu:37-85 + Z2D + gthm_scale :88-98): the
+// reference draws per half-spectrum mode (i,j,k) and component d the complex
+// value (xi1*re, xi2*im), re = im = coef/sqrt(2) (re = coef, im = 0 for a
+// self-conjugate mode), coef = sqrt(sum_a 2(1-cos theta_a))/dx, and maps it
+// to real space with an unnormalised c2r scaled by sqrt(2 kB T dt mu / N).
+// In the x-frequency planes i = 0 and i = nx/2 the c2r keeps only
+// (g(i,j,k) + conj g(i,-j,-k))/2, so that is what is added here; the DFT of
+// the real field is N times that, and u* receives f/rho.
+//
+// Projection (cuda/SELM_Eulerian_Period.cu:109-125): with D_a = e^{i theta_a}-1
+// (forward difference), G_a = 1-e^{-i theta_a} (backward difference) and
+// lambda = sum_a 2(cos theta_a - 1) = sum_a G_a D_a, the update
+// u = u* - G lambda^{-1} D.u* is the DFT image of div -> Poisson -> grad;
+// the dx factors cancel.  The k=0 mode is zeroed (p(0)=0, then mean removal).
 template <typename T, int N>
-__device__ __forceinline__ void fx_c2r_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
-    constexpr int BX = FastCfg<T>::BX, P = BX / 2, S = P + 1, NT = FastCfg<T>::NTX;
-    constexpr int XC = N / 2 + 1;
-    const long long rows = 3LL * R.n[1] * R.n[2];
-    const long long row0 = item * BX;
-    const cplx<T> *in = (const cplx<T> *)R.spec;
-    // Z_e = A_e + i B_e with the Hermitian extension; only the real parts of
-    // the DC and Nyquist coefficients enter (c2r semantics)
+__device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+    constexpr int Q = FastCfg<T>::Q, S = 3 * Q + 1, NT = FastCfg<T>::NTZ;
+    const int ny = R.n[1], xc = R.xc;
+    const int nyb = (ny + Q - 1) / Q;
+    const int jb = (int)(item % nyb);
+    const int kx = (int)(item / nyb);
+    const int j0 = jb * Q;
+    const int jv = min(Q, ny - j0);
+    cplx<T> *C = (cplx<T> *)R.spec2;
+    // sequence s = d*Q + q <-> row C[d][kx][j0+q][*]
+    for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
+        const int s = idx / N, e = idx - s * N;
+        const int d = s / Q, q = s - d * Q;
+        a[e * S + s] = q < jv ? C[(((long long)d * xc + kx) * ny + j0 + q) * N + e] : czero<T>();
+    }
+    __syncthreads();
+    auto nold = [](int, int) { return cplx<T>{}; };
+    auto nost = [](int, int, cplx<T>) {};
+    fft_run<T, N, S, NT, false, false>(a, tw, -1, nold, nost);
+
+    const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
+    const T invN = (T)(1.0 / (double)R.N);
+    const int i = kx;
+    const long long Nc = R.Nc;
+    const cplx<T> *nb = (const cplx<T> *)R.nbuf + ((long long)kx * ny + j0) * N;
+    // consecutive threads take consecutive kz of one row: the noise rows are read contiguously
+    for (int idx = threadIdx.x; idx < N * Q; idx += NT) {
+        const int q = idx / N, e = idx - q * N;
+        if (q >= jv) continue;
+        const int j = j0 + q, k = e;
+        cplx<T> w[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) w[d] = a[e * S + d * Q + q];
+        const T lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
+        if (R.noise) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) w[d] = cadd(w[d], nb[d * Nc + (long long)q * N + e]);
+        }
+        if (i == 0 && j == 0 && k == 0) {
+            if (R.diag) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) R.diag[d] = (double)w[d].x;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = czero<T>();
+            continue;
+        }
+        const cplx<T> D0 = mk<T>(kxt[4 * i + 1], kxt[4 * i + 2]);
+        const cplx<T> D1 = mk<T>(kyt[4 * j + 1], kyt[4 * j + 2]);
+        const cplx<T> D2 = mk<T>(kzt[4 * k + 1], kzt[4 * k + 2]);
+        const cplx<T> sdiv = cadd(cadd(cmul(D0, w[0]), cmul(D1, w[1])), cmul(D2, w[2]));
+        const cplx<T> qv = mk<T>(sdiv.x / lam, sdiv.y / lam);
+        const cplx<T> G[3] = {mk<T>(-D0.x, D0.y), mk<T>(-D1.x, D1.y), mk<T>(-D2.x, D2.y)};
+#pragma unroll
+        for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = cscale(csub(w[d], cmul(G[d], qv)), invN);
+    }
+    __syncthreads();
+    fft_run<T, N, S, NT, false, false>(a, tw, +1, nold, nost);
+    for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
+        const int s = idx / N, e = idx - s * N;
+        const int d = s / Q, q = s - d * Q;
+        if (q < jv) C[(((long long)d * xc + kx) * ny + j0 + q) * N + e] = a[e * S + s];
+    }
+}
+
+// ------------------------------------------------------------------ Xi
+// Workgroup: component d, plane z, rows y0 .. y0+RB-1.  Z_e = A_e + i B_e
+// with the Hermitian extension; only the real parts of the DC and Nyquist
+// coefficients enter (c2r semantics).  Writes u^{n+1}.
+template <typename T, int N>
+__device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+    constexpr int RB = FastCfg<T>::RB, P = RB / 2, S = P + 1, NT = FastCfg<T>::NT, XC = N / 2 + 1;
+    constexpr int SS = RB + 1;
+    const int ny = R.n[1], nz = R.n[2];
+    const int nyb = (ny + RB - 1) / RB;
+    const int yb = (int)(item % nyb);
+    const long long dz = item / nyb;
+    const int z = (int)(dz % nz), d = (int)(dz / nz);
+    const int y0 = yb * RB;
+    cplx<T> *stg = a + N * S;  // staging [kx][rl], stride SS
+    const cplx<T> *in = (const cplx<T> *)R.spec + ((long long)d * nz + z) * XC * ny;
+    for (int idx = threadIdx.x; idx < XC * RB; idx += NT) {
+        const int k = idx / RB, rl = idx - k * RB;
+        const int y = y0 + rl;
+        stg[k * SS + rl] = y < ny ? in[(long long)k * ny + y] : czero<T>();
+    }
+    __syncthreads();
     for (int idx = threadIdx.x; idx < P * N; idx += NT) {
         const int p = idx / N, e = idx - p * N;
-        const long long ra = row0 + 2 * p, rb = ra + 1;
         const int src = e < XC ? e : N - e;
-        cplx<T> A = ra < rows ? in[ra * XC + src] : mk<T>((T)0, (T)0);
-        cplx<T> B = rb < rows ? in[rb * XC + src] : mk<T>((T)0, (T)0);
+        cplx<T> A = stg[src * SS + 2 * p], B = stg[src * SS + 2 * p + 1];
         if (e == 0 || 2 * e == N) {
             A.y = (T)0;
             B.y = (T)0;
@@ -108,191 +261,75 @@ __device__ __forceinline__ void fx_c2r_body(const RepDev &R, long long item, cpl
     auto nold = [](int, int) { return cplx<T>{}; };
     auto nost = [](int, int, cplx<T>) {};
     fft_run<T, N, S, NT, false, false>(a, tw, +1, nold, nost);
-    T *out = (T *)R.u;
-    for (int idx = threadIdx.x; idx < BX * N; idx += NT) {
+    T *out = (T *)R.u + ((long long)d * nz + z) * ny * N;
+    for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
         const int rl = idx / N, e = idx - rl * N;
-        const long long row = row0 + rl;
-        if (row >= rows) continue;
-        const cplx<T> z = a[e * S + (rl >> 1)];
-        out[row * N + e] = (rl & 1) ? z.y : z.x;
+        const int y = y0 + rl;
+        if (y >= ny) continue;
+        const cplx<T> v = a[e * S + (rl >> 1)];
+        out[(long long)y * N + e] = (rl & 1) ? v.y : v.x;
     }
 }
 
-// ------------------------------------------------------------------ Y
-template <typename T, int N>
-__device__ __forceinline__ void fy_body(const RepDev &R, long long item, int dir, cplx<T> *tw, cplx<T> *a) {
-    constexpr int S = FastCfg<T>::BY, NT = FastCfg<T>::NTY;
-    const int xc = R.xc;
-    const int nbx = (xc + S - 1) / S;
-    const int xb = (int)(item % nbx);
-    const long long dz = item / nbx;
-    const int x0 = xb * S;
-    cplx<T> *sp = (cplx<T> *)R.spec + dz * N * xc + x0;
-    const int valid = min(S, xc - x0);
-    auto ld = [&](int e, int s) { return s < valid ? sp[(long long)e * xc + s] : mk<T>((T)0, (T)0); };
-    auto st = [&](int e, int s, cplx<T> v) {
-        if (s < valid) sp[(long long)e * xc + s] = v;
-    };
-    fft_run<T, N, S, NT, true, true>(a, tw, dir, ld, st);
-}
-
-// ------------------------------------------------------------------ Z + projection
-template <typename T, int N>
-__device__ __forceinline__ void fz_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
-    constexpr int BZ = FastCfg<T>::BZ, S = 3 * BZ, NT = FastCfg<T>::NTZ;
-    const int nx = R.n[0], ny = R.n[1], xc = R.xc;
-    const long long ncol = (long long)ny * xc;
-    const long long c0 = item * BZ;
-    cplx<T> *sp = (cplx<T> *)R.spec;
-    auto ld = [&](int e, int s) {
-        const int d = s / BZ, c = s - d * BZ;
-        const long long col = c0 + c;
-        return col < ncol ? sp[((long long)d * N + e) * ncol + col] : mk<T>((T)0, (T)0);
-    };
-    auto st = [&](int e, int s, cplx<T> v) {
-        const int d = s / BZ, c = s - d * BZ;
-        const long long col = c0 + c;
-        if (col < ncol) sp[((long long)d * N + e) * ncol + col] = v;
-    };
-    auto nold = [](int, int) { return cplx<T>{}; };
-    auto nost = [](int, int, cplx<T>) {};
-    fft_run<T, N, S, NT, true, false>(a, tw, -1, ld, nost);
-
-    const T *kx = (const T *)R.ktab[0], *ky = (const T *)R.ktab[1], *kz = (const T *)R.ktab[2];
-    const T invN = (T)(1.0 / (double)R.N);
-    const uint32_t step = (uint32_t)(R.ts[0] + 1);
-    const T amp = (T)R.noise_amp;
-    for (int idx = threadIdx.x; idx < N * BZ; idx += NT) {
-        const int e = idx / BZ, c = idx - e * BZ;
-        const long long col = c0 + c;
-        if (col >= ncol) continue;
-        const int j = (int)(col / xc), i = (int)(col - (long long)j * xc), k = e;
-        cplx<T> w[3];
+// ------------------------------------------------------------------ thermal forcing
+// Thread per mode, in the row order of C ([kx][ky][kz], kz fastest).  Runs on
+// a forked stream while the beads are updated and spread.
+template <typename T>
+__global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, int nrep,
+                                               const long long *__restrict__ start, long long total) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int r = find_rep(start, nrep, g);
+    const RepDev &R = reps[r];
+    if (!R.noise) return;
+    const long long m = g - start[r];
+    const int ny = R.n[1], nz = R.n[2];
+    const int k = (int)(m % nz);
+    const long long ij = m / nz;
+    const int j = (int)(ij % ny), i = (int)(ij / ny);
+    const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
+    const T lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
+    cplx<T> gv[3];
+    thermal_mode<T>(R, i, j, k, lam, (uint32_t)(R.ts[0] + 1), gv);
+    cplx<T> *nb = (cplx<T> *)R.nbuf;
 #pragma unroll
-        for (int d = 0; d < 3; d++) w[d] = a[e * S + d * BZ + c];
-        const T lam = (kx[4 * i] + ky[4 * j]) + kz[4 * k];
-        if (R.noise) {
-            const int k1 = (N - k) % N, j1 = (ny - j) % ny, i1 = (nx - i) % nx;
-            const bool selfconj = (k1 == k && j1 == j && i1 == i);
-            const T coef = sqrt(-lam * (T)R.inv_dxsq);
-            const T re = selfconj ? coef : coef / sqrt((T)2);
-            const T im = selfconj ? (T)0 : re;
-            const bool plane = (i == 0) || (2 * i == nx);
-            const uint32_t nmode = (uint32_t)((long long)k * ncol + col);
-            const uint32_t npart = (uint32_t)(((long long)k1 * ny + j1) * xc + i);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                T z0, z1;
-                Normal<T>::pair(R.seed, kStreamFluid, step, nmode, (uint32_t)d, z0, z1);
-                cplx<T> gk = mk<T>(z0 * re, z1 * im);
-                if (plane) {
-                    T y0, y1;
-                    Normal<T>::pair(R.seed, kStreamFluid, step, npart, (uint32_t)d, y0, y1);
-                    gk = mk<T>((gk.x + y0 * re) * (T)0.5, (gk.y - y1 * im) * (T)0.5);
-                }
-                w[d] = cadd(w[d], cscale(gk, amp));
-            }
-        }
-        if (i == 0 && j == 0 && k == 0) {
-            if (R.diag) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) R.diag[d] = (double)w[d].x;
-            }
-#pragma unroll
-            for (int d = 0; d < 3; d++) a[e * S + d * BZ + c] = mk<T>((T)0, (T)0);
-            continue;
-        }
-        const cplx<T> D0 = mk<T>(kx[4 * i + 1], kx[4 * i + 2]);
-        const cplx<T> D1 = mk<T>(ky[4 * j + 1], ky[4 * j + 2]);
-        const cplx<T> D2 = mk<T>(kz[4 * k + 1], kz[4 * k + 2]);
-        const cplx<T> sdiv = cadd(cadd(cmul(D0, w[0]), cmul(D1, w[1])), cmul(D2, w[2]));
-        const cplx<T> q = mk<T>(sdiv.x / lam, sdiv.y / lam);
-        const cplx<T> G[3] = {mk<T>(-D0.x, D0.y), mk<T>(-D1.x, D1.y), mk<T>(-D2.x, D2.y)};
-#pragma unroll
-        for (int d = 0; d < 3; d++) a[e * S + d * BZ + c] = cscale(csub(w[d], cmul(G[d], q)), invN);
-    }
-    __syncthreads();
-    fft_run<T, N, S, NT, false, true>(a, tw, +1, nold, st);
+    for (int d = 0; d < 3; d++) nb[d * R.Nc + m] = gv[d];
 }
 
 // ------------------------------------------------------------------ kernels
-#define SELM_CASE_X(n) \
-    case n: fx_r2c_body<T, n>(R, item, tw, a); break;
-template <typename T>
-__global__ __launch_bounds__(FastCfg<T>::NTX) void k_fx_r2c(const RepDev *__restrict__ reps, int nrep,
-                                                            const long long *__restrict__ start,
-                                                            long long total, int twmax) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = xcd_remap(blockIdx.x, gridDim.x);
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const long long item = g - start[r];
-    cplx<T> *tw = (cplx<T> *)smem;
-    cplx<T> *a = tw + twmax;
-    tw_to_lds<T>(tw, R.tw[0], R.n[0]);
-    switch (R.n[0]) { SELM_FFT_FOR_EACH_N(SELM_CASE_X) default: break; }
-}
-#undef SELM_CASE_X
+#define SELM_FFT_KERNEL(kname, axis, CASE, NTH)                                                       \
+    template <typename T>                                                                             \
+    __global__ __launch_bounds__(FastCfg<T>::NTH) void kname(const RepDev *__restrict__ reps, int nrep, \
+                                                            const long long *__restrict__ start,      \
+                                                            long long total, int twmax) {             \
+        extern __shared__ __align__(16) unsigned char smem[];                                         \
+        const long long g = xcd_remap(blockIdx.x, gridDim.x);                                         \
+        if (g >= total) return;                                                                       \
+        const int r = find_rep(start, nrep, g);                                                       \
+        const RepDev &R = reps[r];                                                                    \
+        const long long item = g - start[r];                                                          \
+        cplx<T> *tw = (cplx<T> *)smem;                                                                \
+        cplx<T> *a = tw + twmax;                                                                      \
+        tw_to_lds<T>(tw, R.tw[axis], R.n[axis]);                                                      \
+        switch (R.n[axis]) { SELM_FFT_FOR_EACH_N(CASE) default: break; }                              \
+    }
 
-#define SELM_CASE_XI(n) \
-    case n: fx_c2r_body<T, n>(R, item, tw, a); break;
-template <typename T>
-__global__ __launch_bounds__(FastCfg<T>::NTX) void k_fx_c2r(const RepDev *__restrict__ reps, int nrep,
-                                                            const long long *__restrict__ start,
-                                                            long long total, int twmax) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = xcd_remap(blockIdx.x, gridDim.x);
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const long long item = g - start[r];
-    cplx<T> *tw = (cplx<T> *)smem;
-    cplx<T> *a = tw + twmax;
-    tw_to_lds<T>(tw, R.tw[0], R.n[0]);
-    switch (R.n[0]) { SELM_FFT_FOR_EACH_N(SELM_CASE_XI) default: break; }
-}
-#undef SELM_CASE_XI
-
-#define SELM_CASE_Y(n) \
-    case n: fy_body<T, n>(R, item, dir, tw, a); break;
-template <typename T>
-__global__ __launch_bounds__(FastCfg<T>::NTY) void k_fy(const RepDev *__restrict__ reps, int nrep,
-                                                        const long long *__restrict__ start, long long total,
-                                                        int dir, int twmax) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = xcd_remap(blockIdx.x, gridDim.x);
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const long long item = g - start[r];
-    cplx<T> *tw = (cplx<T> *)smem;
-    cplx<T> *a = tw + twmax;
-    tw_to_lds<T>(tw, R.tw[1], R.n[1]);
-    __syncthreads();
-    switch (R.n[1]) { SELM_FFT_FOR_EACH_N(SELM_CASE_Y) default: break; }
-}
-#undef SELM_CASE_Y
-
+#define SELM_CASE_XF(n) \
+    case n: xf_body<T, n>(R, item, tw, a); break;
+#define SELM_CASE_YF(n) \
+    case n: y_body<T, n, true>(R, item, tw, a); break;
+#define SELM_CASE_YI(n) \
+    case n: y_body<T, n, false>(R, item, tw, a); break;
 #define SELM_CASE_Z(n) \
-    case n: fz_body<T, n>(R, item, tw, a); break;
-template <typename T>
-__global__ __launch_bounds__(FastCfg<T>::NTZ) void k_fz_proj(const RepDev *__restrict__ reps, int nrep,
-                                                             const long long *__restrict__ start,
-                                                             long long total, int twmax) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const long long g = xcd_remap(blockIdx.x, gridDim.x);
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const long long item = g - start[r];
-    cplx<T> *tw = (cplx<T> *)smem;
-    cplx<T> *a = tw + twmax;
-    tw_to_lds<T>(tw, R.tw[2], R.n[2]);
-    __syncthreads();
-    switch (R.n[2]) { SELM_FFT_FOR_EACH_N(SELM_CASE_Z) default: break; }
-}
-#undef SELM_CASE_Z
+    case n: z_body<T, n>(R, item, tw, a); break;
+#define SELM_CASE_XI(n) \
+    case n: xi_body<T, n>(R, item, tw, a); break;
+
+SELM_FFT_KERNEL(k_xf, 0, SELM_CASE_XF, NT)
+SELM_FFT_KERNEL(k_yf, 1, SELM_CASE_YF, NT)
+SELM_FFT_KERNEL(k_zp, 2, SELM_CASE_Z, NTZ)
+SELM_FFT_KERNEL(k_yi, 1, SELM_CASE_YI, NT)
+SELM_FFT_KERNEL(k_xi, 0, SELM_CASE_XI, NT)
 
 // ------------------------------------------------------------------ host side
 static inline bool fast_len(int n) {
@@ -307,52 +344,64 @@ static inline bool fast_len(int n) {
 
 bool fft_fast_supported(const int n[3]) { return fast_len(n[0]) && fast_len(n[1]) && fast_len(n[2]); }
 
-void fft_fast_blocks(bool fp64, int *bx, int *by, int *bz) {
-    if (fp64) {
-        *bx = FastCfg<double>::BX;
-        *by = FastCfg<double>::BY;
-        *bz = FastCfg<double>::BZ;
-    } else {
-        *bx = FastCfg<float>::BX;
-        *by = FastCfg<float>::BY;
-        *bz = FastCfg<float>::BZ;
-    }
+// workgroups of each pass for one replica
+void fft_fast_items(bool fp64, const int n[3], long long *ix, long long *iy, long long *iz) {
+    const int RB = fp64 ? FastCfg<double>::RB : FastCfg<float>::RB;
+    const int Q = fp64 ? FastCfg<double>::Q : FastCfg<float>::Q;
+    const long long xc = n[0] / 2 + 1;
+    *ix = 3LL * n[2] * ((n[1] + RB - 1) / RB);
+    *iy = 3LL * xc * ((n[2] + RB - 1) / RB);
+    *iz = xc * ((n[1] + Q - 1) / Q);
 }
 
-// LDS bytes of each fast pass for edges n (twiddle slots twmax first)
+// LDS bytes of each pass for edges n (twiddle slots twmax first)
 void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, size_t *lz) {
     const size_t csz = fp64 ? 16 : 8;
-    int BX, BY, BZ;
-    fft_fast_blocks(fp64, &BX, &BY, &BZ);
-    *lx = ((size_t)twmax + (size_t)(BX / 2 + 1) * n[0]) * csz;
-    *ly = ((size_t)twmax + (size_t)BY * n[1]) * csz;
-    *lz = ((size_t)twmax + (size_t)3 * BZ * n[2]) * csz;
+    const int RB = fp64 ? FastCfg<double>::RB : FastCfg<float>::RB;
+    const int Q = fp64 ? FastCfg<double>::Q : FastCfg<float>::Q;
+    const size_t xc = n[0] / 2 + 1;
+    const size_t xf = (size_t)(RB / 2 + 1) * n[0];
+    const size_t xi = xf + xc * (RB + 1);
+    *lx = ((size_t)twmax + std::max(xf, xi)) * csz;
+    *ly = ((size_t)twmax + (size_t)(RB + 1) * n[1]) * csz;
+    *lz = ((size_t)twmax + (size_t)(3 * Q + 1) * n[2]) * csz;
 }
 
 static inline unsigned pad8(long long n) { return (unsigned)((n + 7) & ~7LL); }
 
 template <typename T>
 static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
+    const int NT = FastCfg<T>::NT;
     switch (stage) {
         case kStageFftX:
-            hipLaunchKernelGGL(k_fx_r2c<T>, dim3(pad8(L.tot_fx)), dim3(FastCfg<T>::NTX), L.lds_fx, st, L.reps,
-                               L.nrep, L.st_fx, L.tot_fx, L.twmax);
+            hipLaunchKernelGGL(k_xf<T>, dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep, L.st_fx,
+                               L.tot_fx, L.twmax);
             break;
         case kStageFftY:
-        case kStageFftYi:
-            hipLaunchKernelGGL(k_fy<T>, dim3(pad8(L.tot_fy)), dim3(FastCfg<T>::NTY), L.lds_fy, st, L.reps,
-                               L.nrep, L.st_fy, L.tot_fy, stage == kStageFftY ? -1 : +1, L.twmax);
+            hipLaunchKernelGGL(k_yf<T>, dim3(pad8(L.tot_fy)), dim3(NT), L.lds_fy, st, L.reps, L.nrep, L.st_fy,
+                               L.tot_fy, L.twmax);
             break;
         case kStageFftZ:
-            hipLaunchKernelGGL(k_fz_proj<T>, dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
-                               L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
+            hipLaunchKernelGGL(k_zp<T>, dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st, L.reps, L.nrep, L.st_fz,
+                               L.tot_fz, L.twmax);
+            break;
+        case kStageFftYi:
+            hipLaunchKernelGGL(k_yi<T>, dim3(pad8(L.tot_fy)), dim3(NT), L.lds_fy, st, L.reps, L.nrep, L.st_fy,
+                               L.tot_fy, L.twmax);
             break;
         case kStageFftXi:
-            hipLaunchKernelGGL(k_fx_c2r<T>, dim3(pad8(L.tot_fx)), dim3(FastCfg<T>::NTX), L.lds_fx, st, L.reps,
-                               L.nrep, L.st_fx, L.tot_fx, L.twmax);
+            hipLaunchKernelGGL(k_xi<T>, dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep, L.st_fx,
+                               L.tot_fx, L.twmax);
             break;
         default: break;
     }
+}
+
+void launch_noise(const StepLaunch &L, hipStream_t st) {
+    if (L.tot_modes <= 0) return;
+    const unsigned nb = (unsigned)((L.tot_modes + 255) / 256);
+    if (L.fp64) hipLaunchKernelGGL(k_noise<double>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes);
+    else hipLaunchKernelGGL(k_noise<float>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes);
 }
 
 void launch_fft_fast(const StepLaunch &L, int stage, hipStream_t st) {

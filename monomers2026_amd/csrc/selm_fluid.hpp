@@ -48,4 +48,41 @@ __device__ __forceinline__ T ustar_at(const RepDev &R, int d, int x, int y, int 
     return uc + ((T)R.mu * lap * (T)R.dt + f) / (T)R.rho;
 }
 
+// Thermal forcing of mode (i, j, k) as it enters u*'s spectrum, for the
+// three components.  The reference (cuda/SELM_Kernel.cu:37-85, then Z2D and
+// gthm_scale :88-98) draws per half-spectrum mode and component the complex
+// value (xi1*re, xi2*im) with re = im = coef/sqrt(2) (re = coef, im = 0 for a
+// self-conjugate mode), coef = sqrt(sum_a 2(1-cos theta_a))/dx, and maps it to
+// real space with an unnormalised c2r scaled by sqrt(2 kB T dt mu / N).  In the
+// x-frequency planes i = 0 and i = nx/2 the c2r keeps only
+// (g(i,j,k) + conj g(i,-j,-k))/2, so that is the value returned; the DFT of
+// the real field is N times it, and u* receives f/rho (noise_amp).
+// lam = sum_a (lambda table)_a as used by the projection.
+template <typename T>
+__device__ __forceinline__ void thermal_mode(const RepDev &R, int i, int j, int k, T lam, uint32_t step,
+                                             cplx<T> g[3]) {
+    const int nx = R.n[0], ny = R.n[1], nz = R.n[2], xc = R.xc;
+    const int k1 = (nz - k) % nz, j1 = (ny - j) % ny, i1 = (nx - i) % nx;
+    const bool selfconj = (k1 == k && j1 == j && i1 == i);
+    const bool plane = (i == 0) || (2 * i == nx);
+    const T coef = sqrt(-lam * (T)R.inv_dxsq);
+    const T re = selfconj ? coef : coef / sqrt((T)2);
+    const T im = selfconj ? (T)0 : re;
+    const T amp = (T)R.noise_amp;
+    const uint32_t nmode = (uint32_t)(((long long)k * ny + j) * xc + i);
+    const uint32_t npart = (uint32_t)(((long long)k1 * ny + j1) * xc + i);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        T z0, z1;
+        Normal<T>::pair(R.seed, kStreamFluid, step, nmode, (uint32_t)d, z0, z1);
+        cplx<T> gk = mk<T>(z0 * re, z1 * im);
+        if (plane) {
+            T y0, y1;
+            Normal<T>::pair(R.seed, kStreamFluid, step, npart, (uint32_t)d, y0, y1);
+            gk = mk<T>((gk.x + y0 * re) * (T)0.5, (gk.y - y1 * im) * (T)0.5);
+        }
+        g[d] = cscale(gk, amp);
+    }
+}
+
 }  // namespace selm

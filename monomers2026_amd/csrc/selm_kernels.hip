@@ -520,26 +520,10 @@ __global__ __launch_bounds__(256) void k_fft_z_proj(const RepDev *__restrict__ r
         const T lx = kx[4 * i], ly = ky[4 * j], lz = kz[4 * k];
         const T lam = (lx + ly) + lz;
         if (R.noise) {
-            const int k1 = (nz - k) % nz, j1 = (ny - j) % ny, i1 = (nx - i) % nx;
-            const bool selfconj = (k1 == k && j1 == j && i1 == i);
-            T coef = sqrt(-lam * (T)R.inv_dxsq);
-            T re = selfconj ? coef : coef / sqrt((T)2);
-            T im = selfconj ? (T)0 : re;
-            const bool plane = (i == 0) || (2 * i == nx);
-            const uint32_t nmode = (uint32_t)((long long)k * ncol + col);
-            const uint32_t npart = (uint32_t)(((long long)k1 * ny + j1) * xc + i);
+            cplx<T> gv[3];
+            thermal_mode<T>(R, i, j, k, lam, step, gv);
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                T z0, z1;
-                Normal<T>::pair(R.seed, kStreamFluid, step, nmode, (uint32_t)d, z0, z1);
-                cplx<T> gk = mk<T>(z0 * re, z1 * im);
-                if (plane) {
-                    T y0, y1;
-                    Normal<T>::pair(R.seed, kStreamFluid, step, npart, (uint32_t)d, y0, y1);
-                    gk = mk<T>((gk.x + y0 * re) * (T)0.5, (gk.y - y1 * im) * (T)0.5);
-                }
-                w[d] = cadd(w[d], cscale(gk, amp));
-            }
+            for (int d = 0; d < 3; d++) w[d] = cadd(w[d], gv[d]);
         }
         if (i == 0 && j == 0 && k == 0) {
             if (R.diag) {
@@ -639,6 +623,9 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
     const RepDev *reps = L.reps;
     const int nrep = L.nrep;
     switch (stage) {
+        case kStageNoise:
+            launch_noise(L, st);
+            break;
         case kStageBead:
             if (L.tot_bead > 0)
                 hipLaunchKernelGGL(k_bead_update<T>, dim3(nblk(L.tot_bead, kBeadsPerBlock)), dim3(kBeadThreads), 0,
@@ -698,8 +685,21 @@ void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
     else launch_stage_t<float>(L, stage, st);
 }
 
+// One step.  The thermal forcing (compile-time-length path) runs on a forked
+// stream concurrently with the bead update, spread and the first two FFT
+// passes, and joins before the Z pass that consumes it.
 void launch_step(const StepLaunch &L, hipStream_t st) {
-    for (int s = 0; s < kNumStages; s++) launch_stage(L, s, st);
+    const bool fork = L.tot_modes > 0;
+    if (fork) {
+        (void)hipEventRecord(L.ev_fork, st);
+        (void)hipStreamWaitEvent(L.st2, L.ev_fork, 0);
+        launch_noise(L, L.st2);
+        (void)hipEventRecord(L.ev_join, L.st2);
+    }
+    for (int s = kStageBead; s < kNumStages; s++) {
+        if (s == kStageFftZ && fork) (void)hipStreamWaitEvent(st, L.ev_join, 0);
+        launch_stage(L, s, st);
+    }
 }
 
 void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st) {

@@ -8,7 +8,8 @@
 namespace selm {
 
 enum Stage {
-    kStageBead = 0,
+    kStageNoise = 0,  // thermal forcing in k-space (forked stream inside a step)
+    kStageBead,
     kStageSpread,
     kStageFftX,  // + viscous predictor u* on the compile-time-length path
 
@@ -30,6 +31,10 @@ struct StepLaunch {
     size_t lds_fx, lds_fy, lds_fz;   // dynamic LDS bytes per pass
     int twmax;                       // LDS twiddle slots (elements) preceding the data
     bool fast;                       // compile-time-length FFT kernels (selm_fft_kernels.hip)
+    const long long *st_modes;       // thermal-forcing work table (half-spectrum modes)
+    long long tot_modes;             // 0 when no replica has thermal forcing (or not fast)
+    hipStream_t st2;                 // forked stream for the thermal forcing
+    hipEvent_t ev_fork, ev_join;
 };
 
 void launch_step(const StepLaunch &L, hipStream_t st);
@@ -41,8 +46,9 @@ size_t spread_rec_bytes(bool fp64);
 
 // selm_fft_kernels.hip
 bool fft_fast_supported(const int n[3]);
-void fft_fast_blocks(bool fp64, int *bx, int *by, int *bz);
+void fft_fast_items(bool fp64, const int n[3], long long *ix, long long *iy, long long *iz);
 void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, size_t *lz);
 void launch_fft_fast(const StepLaunch &L, int stage, hipStream_t st);
+void launch_noise(const StepLaunch &L, hipStream_t st);
 
 }  // namespace selm

@@ -26,8 +26,8 @@ SELM_ERR_POOL_FULL = 6
 PREC_FP64 = 0
 PREC_FP32_BULK = 1
 
-STAGES = ["bead_update", "spread", "ustar_fft_x", "fft_y", "fft_z_projection", "ifft_y", "ifft_x",
-          "bond_forces"]
+STAGES = ["thermal_noise", "bead_update", "spread", "ustar_fft_x", "fft_y", "fft_z_projection", "ifft_y",
+          "ifft_x", "bond_forces"]
 
 EXPORTS = [
     "selm_params_default", "selm_create", "selm_destroy", "selm_last_error", "selm_set_beads",
