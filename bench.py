@@ -77,43 +77,149 @@ def stage_bytes(w, s):
     }
 
 
-def time_steps(sysg, ev, steps, warmup):
+# HBM bytes per launch of each stage's kernel at dilute20k_128 in FP32-bulk,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this script
+# (--fp32-only --copy-calibration 256), each scaled by what the counter reads
+# for a copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the
+# per-kernel table is profiles/pmc_traffic_dilute20k_128.md.
+PMC_TRAFFIC_FP32 = {
+    "thermal_noise": 25827314,
+    "bead_update": 33745575,
+    "spread": 17959920,
+    "ustar_fft_x": 63794839,
+    "fft_y": 51308419,
+    "fft_z_projection": 76951723,
+    "ifft_y": 51303299,
+    "ifft_x": 50879236,
+    "bond_forces": 2584375,
+}
+COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
+
+# roofline classes: which stages are the spread, the FFT and the interpolation
+CLASSES = {
+    "spread": ["spread"],
+    "fft": ["ustar_fft_x", "fft_y", "fft_z_projection", "ifft_y", "ifft_x"],
+    "interp": ["bead_update"],
+}
+
+
+def time_steps(sysg, ev, steps, warmup, runs=1):
+    """Median over `runs` timed regions of `steps` graph replays each."""
     sysg.step(warmup)
     sysg.sync()
-    a, b = ev.new(), ev.new()
     st = sysg.stream
-    t0 = time.perf_counter()
-    ev.record(a, st)
-    sysg.step(steps)
-    ev.record(b, st)
-    ms = ev.elapsed_ms(a, b)
-    wall = time.perf_counter() - t0
-    return ms, wall
+    res = []
+    for _ in range(runs):
+        a, b = ev.new(), ev.new()
+        t0 = time.perf_counter()
+        ev.record(a, st)
+        sysg.step(steps)
+        ev.record(b, st)
+        ms = ev.elapsed_ms(a, b)
+        res.append((ms, time.perf_counter() - t0))
+    res.sort()
+    return res[len(res) // 2]
 
 
-def cpu_baseline(w, budget_s=12.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, budget_s=10.0):
+    """The FP64 oracle (same stage decomposition as the reference's CPU path)
+    timed on this host: all cores (up to 16) for the reported value, and
+    1 thread beside it.  Bounded samples of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
-    O.set_threads(threads)
-    o = O.Oracle(*w.dims, w.lo, w.hi, rho=w.rho, mu=w.mu, kB=w.kB, T=w.T, dt=w.dt, seed=1, noise=True)
-    o.set_beads(w.x, w.v, w.m)
-    o.set_bonds(w.ij, w.K, w.r0)
-    if w.ext_idx is not None:
-        o.set_external_force(w.ext_idx, w.ext_F)
-    o.init()
-    t = time.perf_counter()
-    o.step(1)
-    per = time.perf_counter() - t
-    n = max(2, min(200, int(budget_s / max(per, 1e-3))))
-    t = time.perf_counter()
-    o.step(n)
-    el = time.perf_counter() - t
-    o.close()
-    return {"value": w.M * n / el, "unit": "bead-steps/s", "cores": threads, "kind": "port",
+    ncpu = os.cpu_count() or 1
+    out = {}
+    for threads in (min(16, ncpu), 1):
+        O.set_threads(threads)
+        o = O.Oracle(*w.dims, w.lo, w.hi, rho=w.rho, mu=w.mu, kB=w.kB, T=w.T, dt=w.dt, seed=1, noise=True)
+        o.set_beads(w.x, w.v, w.m)
+        o.set_bonds(w.ij, w.K, w.r0)
+        if w.ext_idx is not None:
+            o.set_external_force(w.ext_idx, w.ext_F)
+        o.init()
+        t = time.perf_counter()
+        o.step(1)
+        per = time.perf_counter() - t
+        n = max(2, min(200, int(budget_s / max(per, 1e-3))))
+        t = time.perf_counter()
+        o.step(n)
+        el = time.perf_counter() - t
+        o.close()
+        out[threads] = (n, el)
+    threads = min(16, ncpu)
+    n, el = out[threads]
+    n1, el1 = out[1]
+    return {"value": round(w.M * n / el, 1), "unit": "bead-steps/s", "cores": threads, "kind": "port",
             "sample": "%d steps of dilute20k_128 (FP64 oracle, OpenMP %d threads) after 1 warm-up step, %.1f s"
                       % (n, threads, el),
-            "ms_per_step": 1e3 * el / n}
+            "ms_per_step": round(1e3 * el / n, 3),
+            "one_thread": {"value": round(w.M * n1 / el1, 1), "ms_per_step": round(1e3 * el1 / n1, 3),
+                           "sample": "%d steps, %.1f s" % (n1, el1)},
+            "host": {"nproc": ncpu, "model": cpu_model()}}
+
+
+def copy_calibration(mib):
+    """5 launches of an elementwise device copy (torch's neg kernel) over mib
+    MiB of float32: each reads and writes exactly mib MiB, the known-size
+    reference for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def ensemble_leg(ev, steps=500, warmup=10):
+    """256 independent systems (edge in {16,24,32,48,64}, one chain of 32-512
+    beads, mu ~ U[0.5, 2]) in one pool, FP32-bulk, noise on: per-step latency
+    of the batched step from HIP events around each step, and throughput."""
+    from monomers2026_amd import selm, workloads
+    ws = workloads.ensemble()
+    fps = []
+    for i, w in enumerate(ws):
+        fps.append(selm.footprint(w.params(selm.PREC_FP32_BULK, True, seed=1000 + i), w.M, w.ij.shape[0]))
+    budget = int(sum(fps) * 1.02) + (64 << 20)
+    pool = selm.Pool(budget, len(ws), precision=selm.PREC_FP32_BULK)
+    for i, w in enumerate(ws):
+        sl = pool.admit(w.params(selm.PREC_FP32_BULK, True, seed=1000 + i), w.x, w.v, w.m, w.ij, w.K, w.r0)
+        if sl is None:
+            raise RuntimeError("ensemble system %d does not fit the pool" % i)
+    pool.step(warmup)
+    pool.sync()
+    st = pool.stream
+    evs = [ev.new() for _ in range(steps + 1)]
+    ev.record(evs[0], st)
+    for k in range(steps):
+        pool.step(1)
+        ev.record(evs[k + 1], st)
+    lat = sorted(ev.elapsed_ms(evs[k], evs[k + 1]) for k in range(steps))
+    total_ms = ev.elapsed_ms(evs[0], evs[-1])
+    used, _, active = pool.stats()
+    pool.close()
+    M = sum(w.M for w in ws)
+    G = sum(w.edge ** 3 for w in ws)
+
+    def pct(q):
+        return lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+
+    return {"systems": active, "beads": M, "grid_points": G, "steps": steps, "hbm_bytes": used,
+            "p50_ms": round(pct(0.5), 5), "p99_ms": round(pct(0.99), 5), "max_ms": round(lat[-1], 5),
+            "bead_steps_per_s": round(M * steps / (total_ms * 1e-3), 1),
+            "grid_point_steps_per_s": round(G * steps / (total_ms * 1e-3), 1)}
 
 
 def main():
@@ -121,8 +227,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--runs", type=int, default=5, help="timed regions; the median is reported")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=50)
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="time only the FP32-bulk steps (no FP64 leg, stage split, ensemble or CPU baseline); "
+                         "for counter runs under rocprofv3 --pmc")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run 5 launches of a plain device copy of MIB MiB (read MIB, write MIB) "
+                         "so that a counter run can scale FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the SELM step of one system runs on one device and "
@@ -134,18 +248,27 @@ def main():
     w = workloads.dilute20k_128()
     ev = HipEvents()
     N = w.edge ** 3
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
 
     # FP32-bulk headline
     s32 = w.build_system(selm.PREC_FP32_BULK, stochastic=True, seed=2026)
-    ms32, wall32 = time_steps(s32, ev, args.steps, args.warmup)
+    ms32, wall32 = time_steps(s32, ev, args.steps, args.warmup, 1 if args.fp32_only else args.runs)
+    if args.fp32_only:
+        s32.close()
+        print(json.dumps({"fp32_only": True, "steps": args.steps, "ms_per_step": round(ms32 / args.steps, 5)}),
+              flush=True)
+        return
+    print("fp32-bulk: %.4f ms/step" % (ms32 / args.steps), file=sys.stderr, flush=True)
     stages32 = s32.stage_times(args.stage_steps)
     s32.close()
     # FP64 path on the same workload
     s64 = w.build_system(selm.PREC_FP64, stochastic=True, seed=2026)
     steps64 = max(1, args.steps // 2)
-    ms64, wall64 = time_steps(s64, ev, steps64, max(1, args.warmup // 2))
+    ms64, wall64 = time_steps(s64, ev, steps64, max(1, args.warmup // 2), max(1, args.runs // 2))
     stages64 = s64.stage_times(args.stage_steps)
     s64.close()
+    print("fp64: %.4f ms/step" % (ms64 / steps64), file=sys.stderr, flush=True)
 
     per32 = ms32 / args.steps
     per64 = ms64 / steps64
@@ -159,8 +282,16 @@ def main():
     b32 = stage_bytes(w, 4)
     achieved = b32[dom] / (avg32[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes": b32[dom], "kernel_ms": round(avg32[dom], 5)}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_FP32[dom],
+            "algorithmic_bytes": b32[dom], "kernel_ms": round(avg32[dom], 5),
+            "copy_ceiling_GBs": COPY_CEILING_GBS}
+    classes = {}
+    for c, names in CLASSES.items():
+        by = sum(b32[k] for k in names)
+        ms = sum(avg32[k] for k in names)
+        gbs = by / (ms * 1e-3) / 1e9
+        classes[c] = {"algorithmic_bytes": by, "ms": round(ms, 5), "GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
     step_bytes = sum(b32.values())
     out = {
         "metric": "bead-steps/s at 128^3 FP32-bulk (GPU), SELM step, 20k beads",
@@ -177,15 +308,21 @@ def main():
         "data": "synthetic: seeded random-walk chains, random-init fluid at rest",
         "config": {"workload": "dilute20k_128", "grid": [128, 128, 128], "beads": w.M, "chains": 200,
                    "beads_per_chain": 100, "external_force_beads": 1000, "noise": True, "dt": w.dt},
+        "timing": "median of %d timed regions of %d steps" % (args.runs, args.steps),
         "grid_point_steps_per_s": round(N * args.steps / (ms32 * 1e-3), 1),
         "fp64": {"value": round(bead_steps64, 1), "ms_per_step": round(per64, 5), "steps": steps64},
         "fp32_over_fp64": round(bead_steps32 / bead_steps64, 3),
         "wall_s": {"fp32": round(wall32, 4), "fp64": round(wall64, 4)},
         "stage_ms_fp32": {k: round(v, 5) for k, v in avg32.items()},
         "stage_ms_fp64": {k: round(v, 5) for k, v in avg64.items()},
+        "class_roofline_fp32": classes,
         "step_algorithmic_GBs_fp32": round(step_bytes / (per32 * 1e-3) / 1e9, 1),
         "roofline": roof,
     }
+    if not args.no_ensemble:
+        out["ensemble256"] = ensemble_leg(ev)
+        print("ensemble: p50 %.4f ms p99 %.4f ms" % (out["ensemble256"]["p50_ms"], out["ensemble256"]["p99_ms"]),
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
