@@ -1,0 +1,61 @@
+/*
+ * selm_host.h -- C ABI of the host-side SELM support library
+ * (libselm_host.so, host code only).
+ *
+ * It exposes the pieces of the deck driver that have no GPU work:
+ * - the Model.SELM reader (cuda/SELM_ParmParse.cuh:14-102, used by
+ *   cuda/SELM.cu:17-48);
+ * - the LAMMPS-subset deck reader and the host conservative force terms
+ *   (what the reference's embedded LAMMPS provides through read_data,
+ *   bond/angle/pair styles, fix addforce and setup_minimal:
+ *   cuda/SELM_Lagrangian_LAMMPS.cu:21-27,118-129);
+ * - the DCD writer (`dump dcd`).
+ *
+ * Status: 0 = OK; otherwise selmh_last_error() holds the message (per thread).
+ */
+#ifndef SELM_HOST_H
+#define SELM_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct selmh_deck selmh_deck;
+
+/* Model.SELM as Selm's constructor reads it (cuda/SELM.cu:17-48):
+ * missing required keys are an error ("Key not found: <key>"). */
+typedef struct selmh_model {
+    double rho, T, mu, KB;
+    int nsteps;
+    int flagStochasticDriving; /* default 0 */
+    int SELM_Seed;             /* default 123456 */
+    int numMeshPtsPerDir[3];
+    int saveSkipSimulationData, writeParticalV, writeParticalF, writeParticalX;
+    char LagrangianType[64], EulerianType[64];
+} selmh_model;
+
+const char *selmh_last_error(void);
+int selmh_model_load(const char *path, selmh_model *out);
+
+/* LAMMPS-subset deck (see monomers2026_amd/csrc/host/lammps_subset.hpp) */
+int selmh_deck_load(const char *script_path, selmh_deck **out);
+void selmh_deck_free(selmh_deck *d);
+int selmh_deck_sizes(const selmh_deck *d, int *natoms, int *nbonds, int *nangles, int *ntypes);
+int selmh_deck_box(const selmh_deck *d, double lo[3], double hi[3], double *dt);
+/* atoms in ascending atom-ID order: x, v [n][3], per-atom mass [n], type [n], id [n]; NULL skips */
+int selmh_deck_atoms(const selmh_deck *d, double *x, double *v, double *m, int *type, int *id);
+/* bonds: ij [nb][2] zero-based, K [nb], r0 [nb]; fext [n][3] from fix addforce */
+int selmh_deck_bonds(const selmh_deck *d, int *ij, double *K, double *r0);
+int selmh_deck_external_force(const selmh_deck *d, double *fext);
+/* all conservative forces at x (bonds, angles, lj/cut, addforce); energy may be NULL */
+int selmh_deck_forces(const selmh_deck *d, const double *x, double *f, double *energy);
+
+/* DCD trajectory writer */
+int selmh_dcd_open(const char *path, int natoms, int every, double dt, void **h);
+int selmh_dcd_frame(void *h, long long step, const double *x, const double box[3]);
+int selmh_dcd_close(void *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SELM_HOST_H */

@@ -12,6 +12,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libselm_hip.so")
+HOST_LIB = os.path.join(PKG, "libselm_host.so")
 DRIVER = os.path.join(PKG, "selm_hip_driver")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -23,8 +24,12 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                 "-Wno-unused-command-line-argument"]
 
 LIB_SOURCES = ["selm_kernels.hip", "selm_fft_kernels.hip", "selm_engine.cpp"]
-DRIVER_SOURCES = ["host/selm_main.cpp", "host/SELM.cpp", "host/SELM_Eulerian_Period_HIP.cpp",
-                  "host/SELM_Lagrangian_Chain_HIP.cpp", "host/lammps_subset.cpp", "host/dcd_writer.cpp"]
+# host-only support library: Model.SELM reader, LAMMPS-subset deck, host
+# force terms, DCD writer (no GPU code; testable without a GPU)
+HOST_SOURCES = ["host/lammps_subset.cpp", "host/dcd_writer.cpp", "host/selm_host_api.cpp"]
+# the deck driver: Selm / Eulerian_Period_HIP / Lagrangian_LAMMPS_HIP over both libraries
+DRIVER_SOURCES = ["host/selm_main.cpp", "host/SELM.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, "-I", os.path.join(CSRC, "host")]
 
 
 def _run(cmd, verbose):
@@ -41,19 +46,25 @@ def build_lib(verbose=False):
     return LIB
 
 
+def build_host(verbose=False):
+    tmp = HOST_LIB + ".tmp"
+    cmd = ["g++"] + HOST_FLAGS + ["-fPIC", "-shared", "-o", tmp] + HOST_SOURCES
+    _run(cmd, verbose)
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
+
+
 def build_driver(verbose=False):
-    srcs = [s for s in DRIVER_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    if not srcs:
-        return None
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", os.path.join(CSRC, "host"),
-           "-o", DRIVER] + srcs + ["-L", PKG, "-lselm_hip", "-Wl,-rpath,$ORIGIN",
-                                   "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = ["g++"] + HOST_FLAGS + ["-o", DRIVER] + DRIVER_SOURCES + [
+        "-L", PKG, "-lselm_host", "-lselm_hip", "-Wl,-rpath,$ORIGIN",
+        "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
     _run(cmd, verbose)
     return DRIVER
 
 
 def build(verbose=False):
     build_lib(verbose)
+    build_host(verbose)
     build_driver(verbose)
 
 

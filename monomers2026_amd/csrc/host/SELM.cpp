@@ -1,0 +1,232 @@
+// SELM.cpp -- Selm driver and the HIP plugin pair (see SELM.hpp).
+#include "SELM.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "parm_parse.hpp"
+
+namespace selm_host {
+
+void Selm::check(int rc, const char *what) const {
+    if (rc == SELM_OK) return;
+    std::string msg = std::string(what) + " failed (status " + std::to_string(rc) + ")";
+    if (ctx) msg += ": " + std::string(selm_last_error(ctx));
+    throw std::runtime_error(msg);
+}
+
+Selm::Selm(const std::string &d) : dir(d) {
+    ParmParse pp;
+    if (!pp.load(dir + "/Model.SELM")) throw std::runtime_error("Unable to open file: " + dir + "/Model.SELM");
+    std::string LagrangianType, EulerianType;
+    pp.get("rho", rho);
+    pp.get("T", T);
+    pp.get("mu", mu);
+    pp.get("nsteps", nsteps);
+    pp.get("KB", KB);
+    pp.get("LagrangianType", LagrangianType);
+    pp.get("EulerianType", EulerianType);
+    pp.query("flagStochasticDriving", flagStochasticDriving);
+    pp.query("SELM_Seed", SELM_Seed);
+    std::string prec;
+    if (pp.query("SELM_Precision", prec) == 0) {
+        if (prec == "fp64") precision = SELM_PREC_FP64;
+        else if (prec == "fp32_bulk") precision = SELM_PREC_FP32_BULK;
+        else throw std::runtime_error("SELM_Precision must be fp64 or fp32_bulk");
+    }
+    pp.query("SELM_F4Quirk", f4_quirk);
+    if (LagrangianType == "Lagrangian_LAMMPS") {
+        auto *lag = new Lagrangian_LAMMPS_HIP(dir + "/Model.LAMMPS_script");
+        lagrangian = lag;
+        lagrangian->selm = this;
+        pp.query("saveSkipSimulationData", lagrangian->saveSkipSimulationData);
+        pp.query("writeParticalV", lagrangian->writeParticalV);
+        pp.query("writeParticalF", lagrangian->writeParticalF);
+        pp.query("writeParticalX", lagrangian->writeParticalX);
+        deltaT = lag->deck.dt;
+    } else {
+        throw std::runtime_error("unsupported LagrangianType: " + LagrangianType);
+    }
+    if (EulerianType == "Eulerian_Period") {
+        eulerian = new Eulerian_Period_HIP();
+        eulerian->selm = this;
+        pp.get("numMeshPtsPerDir", eulerian->dim);
+    } else {
+        throw std::runtime_error("unsupported EulerianType: " + EulerianType);
+    }
+}
+
+Selm::~Selm() {
+    delete eulerian;
+    delete lagrangian;
+    if (ctx) selm_destroy(ctx);
+}
+
+// The system is created from both sides' inputs (grid and box), so the
+// Eulerian side allocates first and the Lagrangian side then uploads the
+// beads and runs selm_init (the reference initialises the Lagrangian first,
+// cuda/SELM.cu:55-58; nothing depends on the order).
+void Selm::initial() {
+    eulerian->initial();
+    lagrangian->initial();
+}
+
+void Selm::compute_f() {
+    eulerian->compute_f();
+    lagrangian->compute_f();
+}
+
+void Selm::run() {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < nsteps; i++) {
+        timestep++;
+        lagrangian->run();
+        eulerian->run();
+        compute_f();
+    }
+    check(selm_sync(ctx), "selm_sync");
+    total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("\n");
+    std::printf("dt: %f\n", deltaT);
+    std::printf("nGpu: 1\n");
+    std::printf("Total time: %fs\n", total_time);
+    std::printf("Total time average: %fs\n", nsteps > 0 ? total_time / nsteps : 0.0);
+}
+
+void Selm::final() {
+    lagrangian->final();
+    eulerian->final();
+}
+
+// ---------------------------------------------------------------------------
+void Eulerian_Period_HIP::initial() {
+    Lagrangian *lag = selm->lagrangian;
+    selm_params p;
+    selm_params_default(&p);
+    for (int a = 0; a < 3; a++) {
+        p.dims[a] = dim[a];
+        p.lo[a] = lag->lo[a];
+        p.hi[a] = lag->hi[a];
+    }
+    p.rho = selm->rho;
+    p.mu = selm->mu;
+    p.kB = selm->KB;
+    p.T = selm->T;
+    p.dt = selm->deltaT;
+    p.seed = (uint64_t)(uint32_t)selm->SELM_Seed;
+    p.stochastic = selm->flagStochasticDriving ? 1 : 0;
+    p.precision = selm->precision;
+    p.f4_quirk = selm->f4_quirk;
+    dx = (p.hi[0] - p.lo[0]) / dim[0];
+    selm_ctx *c = nullptr;
+    const int rc = selm_create(&p, &c);
+    if (rc != SELM_OK) throw std::runtime_error("selm_create failed (status " + std::to_string(rc) + ")");
+    selm->ctx = c;
+}
+
+void Eulerian_Period_HIP::run() { selm->check(selm_step(selm->ctx, 1, nullptr), "selm_step"); }
+
+void Eulerian_Period_HIP::final() {}
+
+// ---------------------------------------------------------------------------
+Lagrangian_LAMMPS_HIP::Lagrangian_LAMMPS_HIP(const std::string &script) : deck(load_deck(script)) {
+    nlocal = deck.natoms;
+    lo = deck.lo;
+    hi = deck.hi;
+}
+
+void Lagrangian_LAMMPS_HIP::initial() {
+    selm_ctx *c = selm->ctx;
+    const int n = deck.natoms;
+    const std::vector<double> m = deck.masses();
+    selm->check(selm_set_beads(c, n, deck.x.data(), deck.v.data(), m.data()), "selm_set_beads");
+    const int nb = (int)deck.bond_type.size();
+    if (nb > 0) {
+        std::vector<int> ij(2 * (size_t)nb);
+        std::vector<double> K(nb), r0(nb);
+        for (int q = 0; q < nb; q++) {
+            ij[2 * q] = deck.bond_i[q];
+            ij[2 * q + 1] = deck.bond_j[q];
+            K[q] = deck.bond_K[deck.bond_type[q]];
+            r0[q] = deck.bond_r0[deck.bond_type[q]];
+        }
+        selm->check(selm_set_bonds(c, nb, ij.data(), K.data(), r0.data()), "selm_set_bonds");
+    }
+    std::vector<int> idx;
+    std::vector<double> F;
+    for (int i = 0; i < n; i++) {
+        const double *f = &deck.fext[3 * (size_t)i];
+        if (f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0) {
+            idx.push_back(i);
+            F.insert(F.end(), f, f + 3);
+        }
+    }
+    if (!idx.empty())
+        selm->check(selm_set_external_force(c, (int)idx.size(), idx.data(), F.data()), "selm_set_external_force");
+    selm->check(selm_init(c), "selm_init");
+    x_.assign(3 * (size_t)n, 0.0);
+    v_.assign(3 * (size_t)n, 0.0);
+    fc_.assign(3 * (size_t)n, 0.0);
+    ft_.assign(3 * (size_t)n, 0.0);
+    selm->check(selm_get_beads(c, x_.data(), v_.data(), fc_.data(), ft_.data()), "selm_get_beads");
+    if (deck.has_host_terms()) {
+        host_forces(deck, x_.data(), fc_.data());
+        selm->check(selm_set_conservative_force(c, fc_.data()), "selm_set_conservative_force");
+    }
+    const double box[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    for (const DumpDcd &dd : deck.dumps) {
+        auto w = std::make_unique<DcdWriter>();
+        const std::string path = dd.file[0] == '/' ? dd.file : selm->dir + "/" + dd.file;
+        if (!w->open(path, n, dd.every, deck.dt, "Written by monomers2026_amd SELM-HIP"))
+            throw std::runtime_error("cannot open " + path);
+        w->frame(0, x_.data(), box);
+        dcd_.push_back(std::move(w));
+    }
+}
+
+void Lagrangian_LAMMPS_HIP::compute_f() {
+    const long long ts = selm->timestep;
+    const bool out = saveSkipSimulationData > 0 && ts % saveSkipSimulationData == 0 &&
+                     (writeParticalX || writeParticalV || writeParticalF);
+    bool dump = false;
+    for (size_t k = 0; k < dcd_.size(); k++) dump |= ts % deck.dumps[k].every == 0;
+    const bool host = deck.has_host_terms();
+    if (!out && !dump && !host) return;
+    selm_ctx *c = selm->ctx;
+    selm->check(selm_get_beads(c, x_.data(), v_.data(), fc_.data(), ft_.data()), "selm_get_beads");
+    if (out) writedata();
+    if (dump) {
+        const double box[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        for (size_t k = 0; k < dcd_.size(); k++)
+            if (ts % deck.dumps[k].every == 0) dcd_[k]->frame(ts, x_.data(), box);
+    }
+    if (host) {
+        host_forces(deck, x_.data(), fc_.data());
+        selm->check(selm_set_conservative_force(c, fc_.data()), "selm_set_conservative_force");
+    }
+}
+
+// particle{F,V,X}.dat, "%g %g %g" per bead, appended (the reference's
+// writedata, cuda/SELM_Lagrangian_LAMMPS.cu:145-175): F is the total force of
+// the last update, V and X the bead state after it.
+void Lagrangian_LAMMPS_HIP::writedata() {
+    const std::string base = selm->dir + "/";
+    auto dump = [&](const char *name, const std::vector<double> &a) {
+        std::FILE *fp = std::fopen((base + name).c_str(), "a");
+        if (!fp) throw std::runtime_error(std::string("cannot open ") + name);
+        for (int i = 0; i < nlocal; i++) std::fprintf(fp, "%g %g %g\n", a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+        std::fclose(fp);
+    };
+    if (writeParticalF) dump("particleF.dat", ft_);
+    if (writeParticalV) dump("particleV.dat", v_);
+    if (writeParticalX) dump("particleX.dat", x_);
+}
+
+void Lagrangian_LAMMPS_HIP::final() {
+    for (auto &w : dcd_) w->close();
+    dcd_.clear();
+}
+
+}  // namespace selm_host

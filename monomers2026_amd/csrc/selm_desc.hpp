@@ -26,6 +26,11 @@ struct RepDev {
     int *tstamp;          // tstamp[tile] == the current step (a bead touched the tile)
     const void *tw[3];    // cplx<T>[n_a]: e^{-2 pi i m / n_a}
     const void *ktab[3];  // per axis, T[4*n_a]: {lam_a, Dre_a, Dim_a, sqrt-part}
+    // FP64 copies used by the compile-time-length FFT kernels, whose arithmetic
+    // is FP64 in both precisions (the same arrays as tw/ktab in FP64; in
+    // FP32-bulk, lam_a = -4 sin^2(theta_a/2))
+    const void *twd[3];
+    const void *ktabd[3];
     RadixPlan plan[3];
     int bx, by, bz;       // rows / columns per workgroup in the X, Y, Z passes
     // ---- tiles for binning / spread

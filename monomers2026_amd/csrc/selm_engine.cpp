@@ -115,6 +115,8 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     b += 2 * al(3 * (size_t)G.N * s);     // u, fsp
     b += 3 * al(3 * (size_t)G.Nc * 2 * s);    // spec, spec2, nbuf
     for (int a = 0; a < 3; a++) b += al((size_t)G.n[a] * 2 * s) + al(4 * (size_t)G.n[a] * s);
+    if (!fp64)  // FP64 twiddle and wavenumber tables of the FFT passes
+        for (int a = 0; a < 3; a++) b += al((size_t)G.n[a] * 2 * 8) + al(4 * (size_t)G.n[a] * 8);
     b += al(8) + al(3 * 8);               // ts, diag
     const size_t tcells = (size_t)G.te[0] * G.te[1] * G.te[2];
     b += al((size_t)G.ntiles * 3 * tcells * s);                   // pool
@@ -169,6 +171,15 @@ static void carve_grid(Replica &R, char *base) {
     for (int a = 0; a < 3; a++) {
         R.D.tw[a] = take((size_t)G.n[a] * 2 * s);
         R.D.ktab[a] = take(4 * (size_t)G.n[a] * s);
+    }
+    for (int a = 0; a < 3; a++) {
+        if (R.fp64) {
+            R.D.twd[a] = R.D.tw[a];
+            R.D.ktabd[a] = R.D.ktab[a];
+        } else {
+            R.D.twd[a] = take((size_t)G.n[a] * 2 * 8);
+            R.D.ktabd[a] = take(4 * (size_t)G.n[a] * 8);
+        }
     }
     R.D.ts = (long long *)take(8);
     R.D.diag = (double *)take(3 * 8);
@@ -282,6 +293,8 @@ static int upload_tables(Replica &R, hipStream_t st, std::string &err) {
             std::vector<float> twf(tw.begin(), tw.end()), ktf(kt.begin(), kt.end());
             HIPCHK(hipMemcpyAsync((void *)R.D.tw[a], twf.data(), twf.size() * 4, hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync((void *)R.D.ktab[a], ktf.data(), ktf.size() * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync((void *)R.D.twd[a], tw.data(), tw.size() * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync((void *)R.D.ktabd[a], kt.data(), kt.size() * 8, hipMemcpyHostToDevice, st));
             HIPCHK(hipStreamSynchronize(st));
         }
     }

@@ -1,7 +1,8 @@
 // selm_fft.hpp -- compile-time-specialised batched FFT for CDNA4.
 //
-// A workgroup transforms S interleaved sequences of length N (element e of
-// sequence s at lds[e*S + s]).  Each Stockham stage keeps its radix-R
+// A workgroup transforms NSEQ interleaved sequences of length N (element e of
+// sequence s at lds[e*S + s]; the stride S >= NSEQ may be padded against LDS
+// bank conflicts).  Each Stockham stage keeps its radix-R
 // butterflies in registers: every thread reads all of its inputs, the
 // workgroup synchronises, and the outputs are written back in place, so one
 // LDS buffer of S*N elements suffices.  The first stage may read its inputs
@@ -9,6 +10,10 @@
 // memory through caller-supplied functors, skipping two LDS round trips.
 // N, S and the thread count are template constants, so the index arithmetic
 // compiles to shifts and multiplies.
+//
+// The arithmetic type C and the LDS element type L may differ: with C =
+// double and L = float, every butterfly and twiddle product is exact to FP64
+// and the data is rounded to FP32 only where it is exchanged through LDS.
 #pragma once
 
 #include "selm_device.hpp"
@@ -116,25 +121,32 @@ SELM_FFT_PLAN(256, 16, 16)
 #define SELM_FFT_FOR_EACH_N(X) \
     X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(80) X(96) X(120) X(128) X(160) X(192) X(256)
 
+template <typename A, typename B>
+__device__ __forceinline__ cplx<A> ccast(cplx<B> v) {
+    return mk<A>((A)v.x, (A)v.y);
+}
+
 // One Stockham stage.  Reads element e of sequence s from `ld(e, s)` (global)
-// or lds, writes through `st(e, s, v)` (global) or lds.
-template <typename T, int N, int S, int NT, int R, int NS, bool FROM_G, bool TO_G, class LD, class ST>
-__device__ __forceinline__ void fft_stage(cplx<T> *lds, const cplx<T> *tw, int dir, const LD &ld, const ST &st) {
-    constexpr int NB = S * (N / R);
+// or lds, writes through `st(e, s, v)` (global) or lds.  Arithmetic in C,
+// LDS elements of type L, twiddles cplx<C>.
+template <typename C, typename L, int N, int S, int NSEQ, int NT, int R, int NS, bool FROM_G, bool TO_G, class LD,
+          class ST>
+__device__ __forceinline__ void fft_stage(cplx<L> *lds, const cplx<C> *tw, int dir, const LD &ld, const ST &st) {
+    constexpr int NB = NSEQ * (N / R);
     constexpr int PER = (NB + NT - 1) / NT;
     constexpr bool FULL = (NB % NT) == 0;
     const int tid = threadIdx.x;
-    cplx<T> v[PER][R];
+    cplx<C> v[PER][R];
 #pragma unroll
     for (int b = 0; b < PER; b++) {
         const int t = tid + b * NT;
         if (FULL || t < NB) {
-            const int s = t % S, j = t / S;
+            const int s = t % NSEQ, j = t / NSEQ;
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int e = j + r * (N / R);
                 if constexpr (FROM_G) v[b][r] = ld(e, s);
-                else v[b][r] = lds[e * S + s];
+                else v[b][r] = ccast<C>(lds[e * S + s]);
             }
         }
     }
@@ -143,44 +155,45 @@ __device__ __forceinline__ void fft_stage(cplx<T> *lds, const cplx<T> *tw, int d
     for (int b = 0; b < PER; b++) {
         const int t = tid + b * NT;
         if (FULL || t < NB) {
-            const int s = t % S, j = t / S;
+            const int s = t % NSEQ, j = t / NSEQ;
             const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    cplx<T> w = tw[(r * k) * (N / (NS * R))];
+                    cplx<C> w = tw[(r * k) * (N / (NS * R))];
                     if (dir > 0) w.y = -w.y;
                     v[b][r] = cmul(v[b][r], w);
                 }
             }
-            dft_r<T, R>(v[b], dir);
+            dft_r<C, R>(v[b], dir);
             const int base = (j / NS) * NS * R + k;
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int e = base + r * NS;
                 if constexpr (TO_G) st(e, s, v[b][r]);
-                else lds[e * S + s] = v[b][r];
+                else lds[e * S + s] = ccast<L>(v[b][r]);
             }
         }
     }
     if constexpr (!TO_G) __syncthreads();
 }
 
-template <typename T, int N, int S, int NT, int I, int NS, bool LDG, bool STG, class LD, class ST>
-__device__ __forceinline__ void fft_stages(cplx<T> *lds, const cplx<T> *tw, int dir, const LD &ld, const ST &st) {
+template <typename C, typename L, int N, int S, int NSEQ, int NT, int I, int NS, bool LDG, bool STG, class LD,
+          class ST>
+__device__ __forceinline__ void fft_stages(cplx<L> *lds, const cplx<C> *tw, int dir, const LD &ld, const ST &st) {
     using P = FftPlan<N>;
     constexpr int R = P::R[I];
     constexpr bool first = I == 0, last = I == P::NST - 1;
-    fft_stage<T, N, S, NT, R, NS, first && LDG, last && STG>(lds, tw, dir, ld, st);
-    if constexpr (!last) fft_stages<T, N, S, NT, I + 1, NS * R, LDG, STG>(lds, tw, dir, ld, st);
+    fft_stage<C, L, N, S, NSEQ, NT, R, NS, first && LDG, last && STG>(lds, tw, dir, ld, st);
+    if constexpr (!last) fft_stages<C, L, N, S, NSEQ, NT, I + 1, NS * R, LDG, STG>(lds, tw, dir, ld, st);
 }
 
 // Full transform.  LDG: first stage reads through ld (else data is in lds);
 // STG: last stage writes through st (else the result is left in lds).
 // The caller synchronises before a first stage that reads lds.
-template <typename T, int N, int S, int NT, bool LDG, bool STG, class LD, class ST>
-__device__ __forceinline__ void fft_run(cplx<T> *lds, const cplx<T> *tw, int dir, const LD &ld, const ST &st) {
-    fft_stages<T, N, S, NT, 0, 1, LDG, STG>(lds, tw, dir, ld, st);
+template <typename C, typename L, int N, int S, int NSEQ, int NT, bool LDG, bool STG, class LD, class ST>
+__device__ __forceinline__ void fft_run(cplx<L> *lds, const cplx<C> *tw, int dir, const LD &ld, const ST &st) {
+    fft_stages<C, L, N, S, NSEQ, NT, 0, 1, LDG, STG>(lds, tw, dir, ld, st);
 }
 
 struct NoLoad {

@@ -15,6 +15,13 @@
 //
 // B lives in spec, C in spec2.  Each kernel switches on the replica's edge
 // length per workgroup, so one launch still covers a heterogeneous batch.
+//
+// Precision: arrays in HBM hold T (float in FP32-bulk, double in FP64).
+// Every butterfly, twiddle product, u* and projection is evaluated in FP64
+// (type C).  The data is rounded to T where it is stored or exchanged
+// through LDS (FastCfg<T>::L).  The FP32-bulk path therefore keeps FP32's
+// HBM traffic while its transforms lose about one FP32 rounding per
+// exchange, rather than one per butterfly level.
 #include <algorithm>
 
 #include "selm_fft.hpp"
@@ -25,22 +32,25 @@ namespace selm {
 
 template <typename T>
 struct FastCfg;
+using C = double;  // arithmetic type of every pass
+
 template <>
 struct FastCfg<float> {
+    using L = float;               // LDS element type
     static constexpr int RB = 32;  // rows per block in Xf/Xi, z per block in Yf/Yi (256 B runs)
     static constexpr int Q = 16;   // (kx, ky) columns per Z workgroup, times 3 components
     static constexpr int NT = 256, NTZ = 512;
 };
 template <>
 struct FastCfg<double> {
+    using L = double;
     static constexpr int RB = 16;
     static constexpr int Q = 8;
     static constexpr int NT = 256, NTZ = 512;
 };
 
-template <typename T>
-__device__ __forceinline__ void tw_to_lds(cplx<T> *dst, const void *src, int n) {
-    const cplx<T> *s = (const cplx<T> *)src;
+__device__ __forceinline__ void tw_to_lds(cplx<C> *dst, const void *src, int n) {
+    const cplx<C> *s = (const cplx<C> *)src;
     for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = s[m];
 }
 
@@ -53,7 +63,9 @@ __device__ __forceinline__ cplx<T> czero() {
 // Workgroup: component d, plane z, rows y0 .. y0+RB-1 of u*; rows are paired
 // into complex sequences (even row = real part, odd row = imaginary part).
 template <typename T, int N>
-__device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+__device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C> *tw,
+                                        cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
     constexpr int RB = FastCfg<T>::RB, P = RB / 2, S = P + 1, NT = FastCfg<T>::NT, XC = N / 2 + 1;
     const int ny = R.n[1], nz = R.n[2];
     const int nyb = (ny + RB - 1) / RB;
@@ -65,18 +77,18 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<T>
     for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
         const int rl = idx / N, e = idx - rl * N;
         const int y = y0 + rl;
-        T v = (T)0;
+        C v = (C)0;
         if (y < ny) {
             int tyz, lyz;
             spread_row(R, y, z, tyz, lyz);
-            v = ustar_at<T>(R, d, e, y, z, tyz, lyz, step);
+            v = ustar_at<T, C>(R, d, e, y, z, tyz, lyz, step);
         }
-        ((T *)&a[e * S + (rl >> 1)])[rl & 1] = v;
+        ((L *)&a[e * S + (rl >> 1)])[rl & 1] = (L)v;
     }
     __syncthreads();
-    auto nold = [](int, int) { return cplx<T>{}; };
-    auto nost = [](int, int, cplx<T>) {};
-    fft_run<T, N, S, NT, false, false>(a, tw, -1, nold, nost);
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, P, NT, false, false>(a, tw, -1, nold, nost);
     // B[d][z][kx][y]: RB consecutive y per kx
     cplx<T> *out = (cplx<T> *)R.spec + ((long long)d * nz + z) * XC * ny;
     for (int idx = threadIdx.x; idx < XC * RB; idx += NT) {
@@ -84,12 +96,12 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<T>
         const int y = y0 + rl;
         if (y >= ny) continue;
         const int p = rl >> 1;
-        const cplx<T> zk = a[k * S + p];
-        const cplx<T> zm = a[((N - k) % N) * S + p];
-        cplx<T> o;
-        if ((rl & 1) == 0) o = mk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
-        else o = mk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
-        out[(long long)k * ny + y] = o;
+        const cplx<C> zk = ccast<C>(a[k * S + p]);
+        const cplx<C> zm = ccast<C>(a[((N - k) % N) * S + p]);
+        cplx<C> o;
+        if ((rl & 1) == 0) o = mk<C>((zk.x + zm.x) * 0.5, (zk.y - zm.y) * 0.5);
+        else o = mk<C>((zk.y + zm.y) * 0.5, (zm.x - zk.x) * 0.5);
+        out[(long long)k * ny + y] = ccast<T>(o);
     }
 }
 
@@ -98,7 +110,9 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<T>
 // Forward: B rows (y contiguous) -> DFT_y -> C[d][kx][ky][z0..] (RB z contiguous).
 // Inverse: C -> iDFT_y -> B rows.
 template <typename T, int N, bool FWD>
-__device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+__device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<C> *tw,
+                                       cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
     constexpr int RB = FastCfg<T>::RB, S = RB + 1, NT = FastCfg<T>::NT;
     const int nz = R.n[2], xc = R.xc;
     const int nzb = (nz + RB - 1) / RB;
@@ -112,27 +126,27 @@ __device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<T> 
     if (FWD) {
         for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
             const int q = idx / N, y = idx - q * N;
-            a[y * S + q] = q < zv ? B[((long long)(z0 + q) * xc + kx) * N + y] : czero<T>();
+            a[y * S + q] = ccast<L>(q < zv ? B[((long long)(z0 + q) * xc + kx) * N + y] : czero<T>());
         }
     } else {
         for (int idx = threadIdx.x; idx < N * RB; idx += NT) {
             const int ky = idx / RB, q = idx - ky * RB;
-            a[ky * S + q] = q < zv ? Cc[(long long)ky * nz + z0 + q] : czero<T>();
+            a[ky * S + q] = ccast<L>(q < zv ? Cc[(long long)ky * nz + z0 + q] : czero<T>());
         }
     }
     __syncthreads();
-    auto nold = [](int, int) { return cplx<T>{}; };
-    auto nost = [](int, int, cplx<T>) {};
-    fft_run<T, N, S, NT, false, false>(a, tw, FWD ? -1 : +1, nold, nost);
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, RB, NT, false, false>(a, tw, FWD ? -1 : +1, nold, nost);
     if (FWD) {
         for (int idx = threadIdx.x; idx < N * RB; idx += NT) {
             const int ky = idx / RB, q = idx - ky * RB;
-            if (q < zv) Cc[(long long)ky * nz + z0 + q] = a[ky * S + q];
+            if (q < zv) Cc[(long long)ky * nz + z0 + q] = ccast<T>(a[ky * S + q]);
         }
     } else {
         for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
             const int q = idx / N, y = idx - q * N;
-            if (q < zv) B[((long long)(z0 + q) * xc + kx) * N + y] = a[y * S + q];
+            if (q < zv) B[((long long)(z0 + q) * xc + kx) * N + y] = ccast<T>(a[y * S + q]);
         }
     }
 }
@@ -156,7 +170,9 @@ __device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<T> 
 // u = u* - G lambda^{-1} D.u* is the DFT image of div -> Poisson -> grad;
 // the dx factors cancel.  The k=0 mode is zeroed (p(0)=0, then mean removal).
 template <typename T, int N>
-__device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+__device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<C> *tw,
+                                       cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
     constexpr int Q = FastCfg<T>::Q, S = 3 * Q + 1, NT = FastCfg<T>::NTZ;
     const int ny = R.n[1], xc = R.xc;
     const int nyb = (ny + Q - 1) / Q;
@@ -164,20 +180,20 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<T> 
     const int kx = (int)(item / nyb);
     const int j0 = jb * Q;
     const int jv = min(Q, ny - j0);
-    cplx<T> *C = (cplx<T> *)R.spec2;
+    cplx<T> *Cs = (cplx<T> *)R.spec2;
     // sequence s = d*Q + q <-> row C[d][kx][j0+q][*]
     for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
         const int s = idx / N, e = idx - s * N;
         const int d = s / Q, q = s - d * Q;
-        a[e * S + s] = q < jv ? C[(((long long)d * xc + kx) * ny + j0 + q) * N + e] : czero<T>();
+        a[e * S + s] = ccast<L>(q < jv ? Cs[(((long long)d * xc + kx) * ny + j0 + q) * N + e] : czero<T>());
     }
     __syncthreads();
-    auto nold = [](int, int) { return cplx<T>{}; };
-    auto nost = [](int, int, cplx<T>) {};
-    fft_run<T, N, S, NT, false, false>(a, tw, -1, nold, nost);
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, 3 * Q, NT, false, false>(a, tw, -1, nold, nost);
 
-    const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
-    const T invN = (T)(1.0 / (double)R.N);
+    const C *kxt = (const C *)R.ktabd[0], *kyt = (const C *)R.ktabd[1], *kzt = (const C *)R.ktabd[2];
+    const C invN = 1.0 / (double)R.N;
     const int i = kx;
     const long long Nc = R.Nc;
     const cplx<T> *nb = (const cplx<T> *)R.nbuf + ((long long)kx * ny + j0) * N;
@@ -186,13 +202,13 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<T> 
         const int q = idx / N, e = idx - q * N;
         if (q >= jv) continue;
         const int j = j0 + q, k = e;
-        cplx<T> w[3];
+        cplx<C> w[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) w[d] = a[e * S + d * Q + q];
-        const T lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
+        for (int d = 0; d < 3; d++) w[d] = ccast<C>(a[e * S + d * Q + q]);
+        const C lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
         if (R.noise) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) w[d] = cadd(w[d], nb[d * Nc + (long long)q * N + e]);
+            for (int d = 0; d < 3; d++) w[d] = cadd(w[d], ccast<C>(nb[d * Nc + (long long)q * N + e]));
         }
         if (i == 0 && j == 0 && k == 0) {
             if (R.diag) {
@@ -200,24 +216,24 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<T> 
                 for (int d = 0; d < 3; d++) R.diag[d] = (double)w[d].x;
             }
 #pragma unroll
-            for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = czero<T>();
+            for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = ccast<L>(czero<C>());
             continue;
         }
-        const cplx<T> D0 = mk<T>(kxt[4 * i + 1], kxt[4 * i + 2]);
-        const cplx<T> D1 = mk<T>(kyt[4 * j + 1], kyt[4 * j + 2]);
-        const cplx<T> D2 = mk<T>(kzt[4 * k + 1], kzt[4 * k + 2]);
-        const cplx<T> sdiv = cadd(cadd(cmul(D0, w[0]), cmul(D1, w[1])), cmul(D2, w[2]));
-        const cplx<T> qv = mk<T>(sdiv.x / lam, sdiv.y / lam);
-        const cplx<T> G[3] = {mk<T>(-D0.x, D0.y), mk<T>(-D1.x, D1.y), mk<T>(-D2.x, D2.y)};
+        const cplx<C> D0 = mk<C>(kxt[4 * i + 1], kxt[4 * i + 2]);
+        const cplx<C> D1 = mk<C>(kyt[4 * j + 1], kyt[4 * j + 2]);
+        const cplx<C> D2 = mk<C>(kzt[4 * k + 1], kzt[4 * k + 2]);
+        const cplx<C> sdiv = cadd(cadd(cmul(D0, w[0]), cmul(D1, w[1])), cmul(D2, w[2]));
+        const cplx<C> qv = mk<C>(sdiv.x / lam, sdiv.y / lam);
+        const cplx<C> G[3] = {mk<C>(-D0.x, D0.y), mk<C>(-D1.x, D1.y), mk<C>(-D2.x, D2.y)};
 #pragma unroll
-        for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = cscale(csub(w[d], cmul(G[d], qv)), invN);
+        for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = ccast<L>(cscale(csub(w[d], cmul(G[d], qv)), invN));
     }
     __syncthreads();
-    fft_run<T, N, S, NT, false, false>(a, tw, +1, nold, nost);
+    fft_run<C, L, N, S, 3 * Q, NT, false, false>(a, tw, +1, nold, nost);
     for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
         const int s = idx / N, e = idx - s * N;
         const int d = s / Q, q = s - d * Q;
-        if (q < jv) C[(((long long)d * xc + kx) * ny + j0 + q) * N + e] = a[e * S + s];
+        if (q < jv) Cs[(((long long)d * xc + kx) * ny + j0 + q) * N + e] = ccast<T>(a[e * S + s]);
     }
 }
 
@@ -226,7 +242,9 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<T> 
 // with the Hermitian extension; only the real parts of the DC and Nyquist
 // coefficients enter (c2r semantics).  Writes u^{n+1}.
 template <typename T, int N>
-__device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<T> *tw, cplx<T> *a) {
+__device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C> *tw,
+                                        cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
     constexpr int RB = FastCfg<T>::RB, P = RB / 2, S = P + 1, NT = FastCfg<T>::NT, XC = N / 2 + 1;
     constexpr int SS = RB + 1;
     const int ny = R.n[1], nz = R.n[2];
@@ -235,39 +253,39 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<T>
     const long long dz = item / nyb;
     const int z = (int)(dz % nz), d = (int)(dz / nz);
     const int y0 = yb * RB;
-    cplx<T> *stg = a + N * S;  // staging [kx][rl], stride SS
+    cplx<L> *stg = a + N * S;  // staging [kx][rl], stride SS
     const cplx<T> *in = (const cplx<T> *)R.spec + ((long long)d * nz + z) * XC * ny;
     for (int idx = threadIdx.x; idx < XC * RB; idx += NT) {
         const int k = idx / RB, rl = idx - k * RB;
         const int y = y0 + rl;
-        stg[k * SS + rl] = y < ny ? in[(long long)k * ny + y] : czero<T>();
+        stg[k * SS + rl] = ccast<L>(y < ny ? in[(long long)k * ny + y] : czero<T>());
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < P * N; idx += NT) {
         const int p = idx / N, e = idx - p * N;
         const int src = e < XC ? e : N - e;
-        cplx<T> A = stg[src * SS + 2 * p], B = stg[src * SS + 2 * p + 1];
+        cplx<L> A = stg[src * SS + 2 * p], B = stg[src * SS + 2 * p + 1];
         if (e == 0 || 2 * e == N) {
-            A.y = (T)0;
-            B.y = (T)0;
+            A.y = (L)0;
+            B.y = (L)0;
         }
         if (e >= XC) {
             A.y = -A.y;
             B.y = -B.y;
         }
-        a[e * S + p] = mk<T>(A.x - B.y, A.y + B.x);
+        a[e * S + p] = mk<L>(A.x - B.y, A.y + B.x);
     }
     __syncthreads();
-    auto nold = [](int, int) { return cplx<T>{}; };
-    auto nost = [](int, int, cplx<T>) {};
-    fft_run<T, N, S, NT, false, false>(a, tw, +1, nold, nost);
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, P, NT, false, false>(a, tw, +1, nold, nost);
     T *out = (T *)R.u + ((long long)d * nz + z) * ny * N;
     for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
         const int rl = idx / N, e = idx - rl * N;
         const int y = y0 + rl;
         if (y >= ny) continue;
-        const cplx<T> v = a[e * S + (rl >> 1)];
-        out[(long long)y * N + e] = (rl & 1) ? v.y : v.x;
+        const cplx<L> v = a[e * S + (rl >> 1)];
+        out[(long long)y * N + e] = (T)((rl & 1) ? v.y : v.x);
     }
 }
 
@@ -297,8 +315,11 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
 }
 
 // ------------------------------------------------------------------ kernels
-#define SELM_FFT_KERNEL(kname, axis, CASE, NTH)                                                       \
-    template <typename T>                                                                             \
+// Each kernel is instantiated per size class NMAX (64, 128, 256) and holds
+// only the lengths up to NMAX.  The register budget is then set by the
+// largest length the batch can hold, not by 256.
+#define SELM_FFT_KERNEL(kname, axis, BODY, NTH)                                                       \
+    template <typename T, int NMAX>                                                                   \
     __global__ __launch_bounds__(FastCfg<T>::NTH) void kname(const RepDev *__restrict__ reps, int nrep, \
                                                             const long long *__restrict__ start,      \
                                                             long long total, int twmax) {             \
@@ -308,22 +329,25 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
         const int r = find_rep(start, nrep, g);                                                       \
         const RepDev &R = reps[r];                                                                    \
         const long long item = g - start[r];                                                          \
-        cplx<T> *tw = (cplx<T> *)smem;                                                                \
-        cplx<T> *a = tw + twmax;                                                                      \
-        tw_to_lds<T>(tw, R.tw[axis], R.n[axis]);                                                      \
-        switch (R.n[axis]) { SELM_FFT_FOR_EACH_N(CASE) default: break; }                              \
+        cplx<C> *tw = (cplx<C> *)smem;                                                                \
+        auto *a = (cplx<typename FastCfg<T>::L> *)(tw + twmax);                                       \
+        tw_to_lds(tw, R.twd[axis], R.n[axis]);                                                        \
+        switch (R.n[axis]) {                                                                          \
+            SELM_FFT_FOR_EACH_N(BODY)                                                                 \
+            default: break;                                                                           \
+        }                                                                                             \
     }
 
 #define SELM_CASE_XF(n) \
-    case n: xf_body<T, n>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) xf_body<T, n>(R, item, tw, a); break;
 #define SELM_CASE_YF(n) \
-    case n: y_body<T, n, true>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) y_body<T, n, true>(R, item, tw, a); break;
 #define SELM_CASE_YI(n) \
-    case n: y_body<T, n, false>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) y_body<T, n, false>(R, item, tw, a); break;
 #define SELM_CASE_Z(n) \
-    case n: z_body<T, n>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) z_body<T, n>(R, item, tw, a); break;
 #define SELM_CASE_XI(n) \
-    case n: xi_body<T, n>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) xi_body<T, n>(R, item, tw, a); break;
 
 SELM_FFT_KERNEL(k_xf, 0, SELM_CASE_XF, NT)
 SELM_FFT_KERNEL(k_yf, 1, SELM_CASE_YF, NT)
@@ -356,45 +380,54 @@ void fft_fast_items(bool fp64, const int n[3], long long *ix, long long *iy, lon
 
 // LDS bytes of each pass for edges n (twiddle slots twmax first)
 void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, size_t *lz) {
-    const size_t csz = fp64 ? 16 : 8;
+    const size_t csz = fp64 ? sizeof(cplx<FastCfg<double>::L>) : sizeof(cplx<FastCfg<float>::L>);
+    const size_t twsz = sizeof(cplx<C>);
     const int RB = fp64 ? FastCfg<double>::RB : FastCfg<float>::RB;
     const int Q = fp64 ? FastCfg<double>::Q : FastCfg<float>::Q;
     const size_t xc = n[0] / 2 + 1;
     const size_t xf = (size_t)(RB / 2 + 1) * n[0];
     const size_t xi = xf + xc * (RB + 1);
-    *lx = ((size_t)twmax + std::max(xf, xi)) * csz;
-    *ly = ((size_t)twmax + (size_t)(RB + 1) * n[1]) * csz;
-    *lz = ((size_t)twmax + (size_t)(3 * Q + 1) * n[2]) * csz;
+    *lx = (size_t)twmax * twsz + std::max(xf, xi) * csz;
+    *ly = (size_t)twmax * twsz + (size_t)(RB + 1) * n[1] * csz;
+    *lz = (size_t)twmax * twsz + (size_t)(3 * Q + 1) * n[2] * csz;
 }
 
 static inline unsigned pad8(long long n) { return (unsigned)((n + 7) & ~7LL); }
 
-template <typename T>
-static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
+template <typename T, int NMAX>
+static void launch_fast_n(const StepLaunch &L, int stage, hipStream_t st) {
     const int NT = FastCfg<T>::NT;
     switch (stage) {
         case kStageFftX:
-            hipLaunchKernelGGL(k_xf<T>, dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep, L.st_fx,
+            hipLaunchKernelGGL((k_xf<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep, L.st_fx,
                                L.tot_fx, L.twmax);
             break;
         case kStageFftY:
-            hipLaunchKernelGGL(k_yf<T>, dim3(pad8(L.tot_fy)), dim3(NT), L.lds_fy, st, L.reps, L.nrep, L.st_fy,
+            hipLaunchKernelGGL((k_yf<T, NMAX>), dim3(pad8(L.tot_fy)), dim3(NT), L.lds_fy, st, L.reps, L.nrep, L.st_fy,
                                L.tot_fy, L.twmax);
             break;
         case kStageFftZ:
-            hipLaunchKernelGGL(k_zp<T>, dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st, L.reps, L.nrep, L.st_fz,
-                               L.tot_fz, L.twmax);
+            hipLaunchKernelGGL((k_zp<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st, L.reps,
+                               L.nrep, L.st_fz, L.tot_fz, L.twmax);
             break;
         case kStageFftYi:
-            hipLaunchKernelGGL(k_yi<T>, dim3(pad8(L.tot_fy)), dim3(NT), L.lds_fy, st, L.reps, L.nrep, L.st_fy,
+            hipLaunchKernelGGL((k_yi<T, NMAX>), dim3(pad8(L.tot_fy)), dim3(NT), L.lds_fy, st, L.reps, L.nrep, L.st_fy,
                                L.tot_fy, L.twmax);
             break;
         case kStageFftXi:
-            hipLaunchKernelGGL(k_xi<T>, dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep, L.st_fx,
+            hipLaunchKernelGGL((k_xi<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep, L.st_fx,
                                L.tot_fx, L.twmax);
             break;
         default: break;
     }
+}
+
+// size class from the largest edge of the batch (twmax >= every edge)
+template <typename T>
+static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
+    if (L.twmax <= 64) launch_fast_n<T, 64>(L, stage, st);
+    else if (L.twmax <= 128) launch_fast_n<T, 128>(L, stage, st);
+    else launch_fast_n<T, 256>(L, stage, st);
 }
 
 void launch_noise(const StepLaunch &L, hipStream_t st) {

@@ -29,8 +29,9 @@ __device__ __forceinline__ void spread_row(const RepDev &R, int y, int z, int &t
 
 // u* = u + (mu * L7(u) * dt + f) / rho with the periodic 7-point Laplacian
 // (cuda/SELM_Kernel.cu:155-195; neighbour order x+, x-, y+, y-, z+, z-).
-template <typename T>
-__device__ __forceinline__ T ustar_at(const RepDev &R, int d, int x, int y, int z, int tyz, int lyz, int step) {
+// Values stored as T, arithmetic in C.
+template <typename T, typename C = T>
+__device__ __forceinline__ C ustar_at(const RepDev &R, int d, int x, int y, int z, int tyz, int lyz, int step) {
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const long long Nyx = (long long)nx * ny;
     const T *ud = (const T *)R.u + d * R.N;
@@ -41,11 +42,12 @@ __device__ __forceinline__ T ustar_at(const RepDev &R, int d, int x, int y, int 
     const long long j0 = (y != 0) ? -(long long)nx : (long long)(ny - 1) * nx;
     const long long k1 = (z != nz - 1) ? Nyx : -(long long)z * Nyx;
     const long long k0 = (z != 0) ? -Nyx : (long long)(nz - 1) * Nyx;
-    const T dxsq = (T)(R.dx * R.dx);
-    const T uc = ud[n];
-    const T lap = (ud[n + i1] + ud[n + i0] + ud[n + j1] + ud[n + j0] + ud[n + k1] + ud[n + k0] - (T)6 * uc) / dxsq;
-    const T f = spread_at<T>(R, d, x, tyz, lyz, step);
-    return uc + ((T)R.mu * lap * (T)R.dt + f) / (T)R.rho;
+    const C dxsq = (C)(R.dx * R.dx);
+    const C uc = (C)ud[n];
+    const C lap = ((C)ud[n + i1] + (C)ud[n + i0] + (C)ud[n + j1] + (C)ud[n + j0] + (C)ud[n + k1] + (C)ud[n + k0] -
+                   (C)6 * uc) / dxsq;
+    const C f = (C)spread_at<T>(R, d, x, tyz, lyz, step);
+    return uc + ((C)R.mu * lap * (C)R.dt + f) / (C)R.rho;
 }
 
 // Thermal forcing of mode (i, j, k) as it enters u*'s spectrum, for the

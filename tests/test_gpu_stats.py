@@ -161,9 +161,11 @@ def test_fp32_bulk_displacement_long(gpu_lib):
               % (n, out[n], np.abs(d64).max()), flush=True)
     s64.close()
     s32.close()
-    # Measured (max-norm): 1.5e-6, 1.4e-5, 2.5e-4; the RMS over beads is
-    # 1.3e-6, 1.4e-5, 7.3e-5.  FP64 interpolation weights and FP64
-    # Box-Muller in the FP32 path change none of these by more than 10%.
-    assert out[100] < 1e-5
-    assert out[1000] < 2e-5
-    assert out[5000] < 5e-4
+    # Measured (max-norm): 2.1e-7, 7.3e-7, 7.5e-6.  With FP32 butterflies
+    # (FP32 arithmetic inside the transforms) they were 1.5e-6, 1.4e-5 and
+    # 2.5e-4: the trajectory difference grows with the per-step rounding of
+    # the fluid update, which the FP64 arithmetic of the passes cuts from
+    # 2.2e-7 to 9e-8 of |u| per step.
+    assert out[100] < 1e-6
+    assert out[1000] < 1e-5
+    assert out[5000] < 5e-5
