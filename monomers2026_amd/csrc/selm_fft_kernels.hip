@@ -59,6 +59,27 @@ __device__ __forceinline__ cplx<T> czero() {
     return mk<T>((T)0, (T)0);
 }
 
+// Element i < COUNT is read by ld(i) and written by st(i, v); thread t takes
+// i = t, t + NT, ...  Each thread issues BATCH reads before it writes any,
+// so the latencies of its global loads overlap instead of adding up.
+// Indices past COUNT re-read element COUNT-1 and are not written.
+template <int NT, int COUNT, int BATCH, class LD, class ST>
+__device__ __forceinline__ void copy_batched(const LD &ld, const ST &st) {
+    constexpr int ITER = (COUNT + NT - 1) / NT;
+    using V = decltype(ld(0));
+#pragma unroll
+    for (int b0 = 0; b0 < ITER; b0 += BATCH) {
+        V buf[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) buf[k] = ld(min((int)threadIdx.x + (b0 + k) * NT, COUNT - 1));
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int i = (int)threadIdx.x + (b0 + k) * NT;
+            if (b0 + k < ITER && i < COUNT) st(i, buf[k]);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ Xf
 // Workgroup: component d, plane z, rows y0 .. y0+RB-1 of u*; rows are paired
 // into complex sequences (even row = real part, odd row = imaginary part).
@@ -74,17 +95,19 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
     const int z = (int)(dz % nz), d = (int)(dz / nz);
     const int y0 = yb * RB;
     const int step = (int)(R.ts[0] + 1);
-    for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
-        const int rl = idx / N, e = idx - rl * N;
-        const int y = y0 + rl;
-        C v = (C)0;
-        if (y < ny) {
+    copy_batched<NT, RB * N, 4>(
+        [&](int idx) {
+            const int rl = idx / N, e = idx - rl * N;
+            const int y = min(y0 + rl, ny - 1);  // rows past the grid are loaded, then zeroed
             int tyz, lyz;
             spread_row(R, y, z, tyz, lyz);
-            v = ustar_at<T, C>(R, d, e, y, z, tyz, lyz, step);
-        }
-        ((L *)&a[e * S + (rl >> 1)])[rl & 1] = (L)v;
-    }
+            const C v = ustar_at<T, C>(R, d, e, y, z, tyz, lyz, step);
+            return y0 + rl < ny ? v : (C)0;
+        },
+        [&](int idx, C v) {
+            const int rl = idx / N, e = idx - rl * N;
+            ((L *)&a[e * S + (rl >> 1)])[rl & 1] = (L)v;
+        });
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
     auto nost = [](int, int, cplx<C>) {};
@@ -124,15 +147,27 @@ __device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<C> 
     cplx<T> *B = (cplx<T> *)R.spec + (long long)d * nz * xc * N;             // [z][kx][y]
     cplx<T> *Cc = (cplx<T> *)R.spec2 + ((long long)d * xc + kx) * N * nz;  // [ky][z]
     if (FWD) {
-        for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
-            const int q = idx / N, y = idx - q * N;
-            a[y * S + q] = ccast<L>(q < zv ? B[((long long)(z0 + q) * xc + kx) * N + y] : czero<T>());
-        }
+        copy_batched<NT, RB * N, 16>(
+            [&](int idx) {
+                const int q = idx / N, y = idx - q * N;
+                const cplx<T> v = B[((long long)(z0 + min(q, zv - 1)) * xc + kx) * N + y];
+                return q < zv ? v : czero<T>();
+            },
+            [&](int idx, cplx<T> v) {
+                const int q = idx / N, y = idx - q * N;
+                a[y * S + q] = ccast<L>(v);
+            });
     } else {
-        for (int idx = threadIdx.x; idx < N * RB; idx += NT) {
-            const int ky = idx / RB, q = idx - ky * RB;
-            a[ky * S + q] = ccast<L>(q < zv ? Cc[(long long)ky * nz + z0 + q] : czero<T>());
-        }
+        copy_batched<NT, N * RB, 16>(
+            [&](int idx) {
+                const int ky = idx / RB, q = idx - ky * RB;
+                const cplx<T> v = Cc[(long long)ky * nz + z0 + min(q, zv - 1)];
+                return q < zv ? v : czero<T>();
+            },
+            [&](int idx, cplx<T> v) {
+                const int ky = idx / RB, q = idx - ky * RB;
+                a[ky * S + q] = ccast<L>(v);
+            });
     }
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
@@ -182,11 +217,17 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<C> 
     const int jv = min(Q, ny - j0);
     cplx<T> *Cs = (cplx<T> *)R.spec2;
     // sequence s = d*Q + q <-> row C[d][kx][j0+q][*]
-    for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
-        const int s = idx / N, e = idx - s * N;
-        const int d = s / Q, q = s - d * Q;
-        a[e * S + s] = ccast<L>(q < jv ? Cs[(((long long)d * xc + kx) * ny + j0 + q) * N + e] : czero<T>());
-    }
+    copy_batched<NT, 3 * Q * N, 16>(
+        [&](int idx) {
+            const int s = idx / N, e = idx - s * N;
+            const int d = s / Q, q = s - d * Q;
+            const cplx<T> v = Cs[(((long long)d * xc + kx) * ny + j0 + min(q, jv - 1)) * N + e];
+            return q < jv ? v : czero<T>();
+        },
+        [&](int idx, cplx<T> v) {
+            const int s = idx / N, e = idx - s * N;
+            a[e * S + s] = ccast<L>(v);
+        });
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
     auto nost = [](int, int, cplx<C>) {};
@@ -255,11 +296,16 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C>
     const int y0 = yb * RB;
     cplx<L> *stg = a + N * S;  // staging [kx][rl], stride SS
     const cplx<T> *in = (const cplx<T> *)R.spec + ((long long)d * nz + z) * XC * ny;
-    for (int idx = threadIdx.x; idx < XC * RB; idx += NT) {
-        const int k = idx / RB, rl = idx - k * RB;
-        const int y = y0 + rl;
-        stg[k * SS + rl] = ccast<L>(y < ny ? in[(long long)k * ny + y] : czero<T>());
-    }
+    copy_batched<NT, XC * RB, 16>(
+        [&](int idx) {
+            const int k = idx / RB, rl = idx - k * RB;
+            const cplx<T> v = in[(long long)k * ny + min(y0 + rl, ny - 1)];
+            return y0 + rl < ny ? v : czero<T>();
+        },
+        [&](int idx, cplx<T> v) {
+            const int k = idx / RB, rl = idx - k * RB;
+            stg[k * SS + rl] = ccast<L>(v);
+        });
     __syncthreads();
     for (int idx = threadIdx.x; idx < P * N; idx += NT) {
         const int p = idx / N, e = idx - p * N;
