@@ -95,6 +95,23 @@ int selm_set_beads(selm_ctx *ctx, int M, const double *x, const double *v, const
  * convention.  ij [nb][2] zero-based bead indices, K [nb], r0 [nb]. */
 int selm_set_bonds(selm_ctx *ctx, int nb, const int *ij, const double *K, const double *r0);
 
+/* LAMMPS `angle_style harmonic` angles (Angles section + angle_coeff), on the
+ * device at every step: E = K (theta - theta0)^2, theta the angle at the
+ * centre bead, theta0 in radians (angle_coeff gives degrees).  ijk [na][3]
+ * zero-based (end, centre, end); replaces the reference's LAMMPS evaluation
+ * in setup_minimal (cuda/SELM_Lagrangian_LAMMPS.cu:122). */
+int selm_set_angles(selm_ctx *ctx, int na, const int *ijk, const double *K, const double *theta0);
+
+/* LAMMPS `pair_style lj/cut` (pair_coeff, special_bonds lj): E = 4 eps
+ * ((sigma/r)^12 - (sigma/r)^6) for r < rc under the minimum image, evaluated
+ * on the device at every step over all bead pairs.  type [M] zero-based,
+ * eps/sigma/rc [ntypes][ntypes] (already mixed), special_lj the weights of
+ * 1-2, 1-3 and 1-4 partners through the bonds of selm_set_bonds (weight 0
+ * drops the pair).  Replaces the reference's LAMMPS evaluation
+ * (cuda/SELM_Lagrangian_LAMMPS.cu:122); ntypes = 0 removes the term. */
+int selm_set_pair_lj(selm_ctx *ctx, int ntypes, const int *type, const double *eps, const double *sigma,
+                     const double *rc, const double special_lj[3]);
+
 /* LAMMPS `fix addforce` with a constant vector on the beads idx[n]: F [n][3]. */
 int selm_set_external_force(selm_ctx *ctx, int n, const int *idx, const double *F);
 
@@ -151,6 +168,18 @@ int selm_mean_mode(selm_ctx *ctx, double out[3]);
 
 /* The ctx's own stream (hipStream_t). */
 void *selm_stream(selm_ctx *ctx);
+
+/* Mean-square displacement of a stored trajectory, with the semantics of the
+ * reference's analysis script (py/Diffusion Coefficient/MSD.py:15-69).
+ * - frames: [nframes][3][natoms] float, the x, y, z records of each DCD frame.
+ * - box[0]: the x edge of the first frame's cell.  Half of it is the minimum
+ *   image threshold X in every dimension, as in the script.
+ * - Origins start at frame floor(nframes / 5).
+ * - msd[0] = 0 and msd[dot] = MSD at lag dot * mult, for dot = 1 .. nlags-1
+ *   (NaN where no origin is left).
+ * Computed on the GPU; terms in FP32 as the script does, sums in FP64. */
+int selm_msd(const float *frames, int nframes, int natoms, const double box[3], int nlags, int mult,
+             double *msd);
 
 /* ---------------------------------------------------------------------------
  * Ensembles and serving (no counterpart in the reference, which runs one

@@ -47,6 +47,11 @@ int selmh_deck_atoms(const selmh_deck *d, double *x, double *v, double *m, int *
 /* bonds: ij [nb][2] zero-based, K [nb], r0 [nb]; fext [n][3] from fix addforce */
 int selmh_deck_bonds(const selmh_deck *d, int *ij, double *K, double *r0);
 int selmh_deck_external_force(const selmh_deck *d, double *fext);
+/* angles: ijk [na][3] zero-based (end, centre, end), K [na], theta0 [na] in radians */
+int selmh_deck_angles(const selmh_deck *d, int *ijk, double *K, double *theta0);
+/* lj/cut after mixing: eps, sigma, rc [ntypes][ntypes]; special_lj [3]; returns
+ * 1 with no pair style (arrays untouched) */
+int selmh_deck_pair_lj(const selmh_deck *d, double *eps, double *sigma, double *rc, double *special_lj);
 /* all conservative forces at x (bonds, angles, lj/cut, addforce); energy may be NULL */
 int selmh_deck_forces(const selmh_deck *d, const double *x, double *f, double *energy);
 

@@ -23,7 +23,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-Wno-unused-command-line-argument"]
 
-LIB_SOURCES = ["selm_kernels.hip", "selm_fft_kernels.hip", "selm_engine.cpp"]
+LIB_SOURCES = ["selm_kernels.hip", "selm_fft_kernels.hip", "selm_analysis.hip", "selm_engine.cpp"]
 # host-only support library: Model.SELM reader, LAMMPS-subset deck, host
 # force terms, DCD writer (no GPU code; testable without a GPU)
 HOST_SOURCES = ["host/lammps_subset.cpp", "host/dcd_writer.cpp", "host/selm_host_api.cpp"]

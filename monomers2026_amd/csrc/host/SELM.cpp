@@ -37,6 +37,7 @@ Selm::Selm(const std::string &d) : dir(d) {
         else throw std::runtime_error("SELM_Precision must be fp64 or fp32_bulk");
     }
     pp.query("SELM_F4Quirk", f4_quirk);
+    pp.query("SELM_HostForces", host_forces_on);
     if (LagrangianType == "Lagrangian_LAMMPS") {
         auto *lag = new Lagrangian_LAMMPS_HIP(dir + "/Model.LAMMPS_script");
         lagrangian = lag;
@@ -154,6 +155,37 @@ void Lagrangian_LAMMPS_HIP::initial() {
         }
         selm->check(selm_set_bonds(c, nb, ij.data(), K.data(), r0.data()), "selm_set_bonds");
     }
+    // angles and lj/cut on the device unless the deck asks for the host hook
+    host_terms_ = deck.has_host_terms() && selm->host_forces_on;
+    if (!host_terms_ && !deck.angle_type.empty()) {
+        const int na = (int)deck.angle_type.size();
+        std::vector<int> ijk(3 * (size_t)na);
+        std::vector<double> K(na), t0(na);
+        for (int q = 0; q < na; q++) {
+            ijk[3 * q] = deck.angle_i[q];
+            ijk[3 * q + 1] = deck.angle_j[q];
+            ijk[3 * q + 2] = deck.angle_k[q];
+            K[q] = deck.angle_K[deck.angle_type[q]];
+            t0[q] = deck.angle_theta0[deck.angle_type[q]];
+        }
+        selm->check(selm_set_angles(c, na, ijk.data(), K.data(), t0.data()), "selm_set_angles");
+    }
+    if (!host_terms_ && deck.pair_style == "lj/cut") {
+        const int nt = deck.ntypes;
+        const size_t T = (size_t)nt + 1;
+        std::vector<int> type(n);
+        for (int i = 0; i < n; i++) type[i] = deck.type[i] - 1;
+        std::vector<double> eps((size_t)nt * nt), sig((size_t)nt * nt), rc((size_t)nt * nt);
+        for (int a = 0; a < nt; a++)
+            for (int b = 0; b < nt; b++) {
+                const size_t k = (size_t)(a + 1) * T + (b + 1);
+                eps[(size_t)a * nt + b] = deck.lj_eps[k];
+                sig[(size_t)a * nt + b] = deck.lj_sigma[k];
+                rc[(size_t)a * nt + b] = deck.lj_rc[k];
+            }
+        selm->check(selm_set_pair_lj(c, nt, type.data(), eps.data(), sig.data(), rc.data(), deck.special_lj),
+                    "selm_set_pair_lj");
+    }
     std::vector<int> idx;
     std::vector<double> F;
     for (int i = 0; i < n; i++) {
@@ -171,7 +203,7 @@ void Lagrangian_LAMMPS_HIP::initial() {
     fc_.assign(3 * (size_t)n, 0.0);
     ft_.assign(3 * (size_t)n, 0.0);
     selm->check(selm_get_beads(c, x_.data(), v_.data(), fc_.data(), ft_.data()), "selm_get_beads");
-    if (deck.has_host_terms()) {
+    if (host_terms_) {
         host_forces(deck, x_.data(), fc_.data());
         selm->check(selm_set_conservative_force(c, fc_.data()), "selm_set_conservative_force");
     }
@@ -192,7 +224,7 @@ void Lagrangian_LAMMPS_HIP::compute_f() {
                      (writeParticalX || writeParticalV || writeParticalF);
     bool dump = false;
     for (size_t k = 0; k < dcd_.size(); k++) dump |= ts % deck.dumps[k].every == 0;
-    const bool host = deck.has_host_terms();
+    const bool host = host_terms_;
     if (!out && !dump && !host) return;
     selm_ctx *c = selm->ctx;
     selm->check(selm_get_beads(c, x_.data(), v_.data(), fc_.data(), ft_.data()), "selm_get_beads");

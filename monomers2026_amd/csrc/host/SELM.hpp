@@ -11,6 +11,9 @@
 //   numMeshPtsPerDir (required for Eulerian_Period)
 //   SELM_Precision fp64 | fp32_bulk   (this build; default fp64)
 //   SELM_F4Quirk 1 | 0                 (this build; default 1, the reference's kernel)
+//   SELM_HostForces 0 | 1              (this build; default 0: harmonic angles and
+//                                       lj/cut run on the device; 1: on the host,
+//                                       handed over every step as LAMMPS does)
 //
 // The type strings are the reference's:
 // - "Lagrangian_LAMMPS" selects Lagrangian_LAMMPS_HIP, which reads
@@ -27,9 +30,10 @@
 // - the particle{X,V,F}.dat files every saveSkipSimulationData steps
 //   (cuda/SELM_Lagrangian_LAMMPS.cu:145-175);
 // - `dump dcd` frames;
-// - the conservative terms that stay on the host (harmonic angles, lj/cut
-//   pairs), added to the device's bond and addforce forces and handed back
-//   through selm_set_conservative_force.
+// - with SELM_HostForces 1, the harmonic angles and lj/cut pairs, added to
+//   the device's bond and addforce forces and handed back through
+//   selm_set_conservative_force.  By default these terms run on the device
+//   with the bonds.
 #pragma once
 
 #include <memory>
@@ -79,6 +83,7 @@ class Selm {
     double KB = 0, T = 0, deltaT = 0, mu = 0, rho = 0;
     int precision = SELM_PREC_FP64;
     int f4_quirk = 1;
+    int host_forces_on = 0;  // SELM_HostForces 1: angles and pairs on the host hook
     Eulerian *eulerian = nullptr;
     Lagrangian *lagrangian = nullptr;
     selm_ctx *ctx = nullptr;  // the system; created by Eulerian::initial
@@ -116,6 +121,7 @@ class Lagrangian_LAMMPS_HIP : public Lagrangian {
   private:
     void writedata();
     std::vector<double> x_, v_, fc_, ft_;
+    bool host_terms_ = false;  // angles / pairs evaluated on the host each step
     std::vector<std::unique_ptr<DcdWriter>> dcd_;
 };
 

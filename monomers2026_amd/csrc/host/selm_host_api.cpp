@@ -134,6 +134,39 @@ int selmh_deck_external_force(const selmh_deck *h, double *fext) {
     return 0;
 }
 
+int selmh_deck_angles(const selmh_deck *h, int *ijk, double *K, double *theta0) {
+    if (!h) return 1;
+    const Deck &d = h->d;
+    for (size_t q = 0; q < d.angle_type.size(); q++) {
+        if (ijk) {
+            ijk[3 * q] = d.angle_i[q];
+            ijk[3 * q + 1] = d.angle_j[q];
+            ijk[3 * q + 2] = d.angle_k[q];
+        }
+        if (K) K[q] = d.angle_K[d.angle_type[q]];
+        if (theta0) theta0[q] = d.angle_theta0[d.angle_type[q]];
+    }
+    return 0;
+}
+
+int selmh_deck_pair_lj(const selmh_deck *h, double *eps, double *sigma, double *rc, double *special_lj) {
+    if (!h) return 1;
+    const Deck &d = h->d;
+    if (d.pair_style != "lj/cut") return 1;
+    const int nt = d.ntypes;
+    const size_t T = (size_t)nt + 1;
+    for (int a = 0; a < nt; a++)
+        for (int b = 0; b < nt; b++) {
+            const size_t k = (size_t)(a + 1) * T + (b + 1), o = (size_t)a * nt + b;
+            if (eps) eps[o] = d.lj_eps[k];
+            if (sigma) sigma[o] = d.lj_sigma[k];
+            if (rc) rc[o] = d.lj_rc[k];
+        }
+    if (special_lj)
+        for (int q = 0; q < 3; q++) special_lj[q] = d.special_lj[q];
+    return 0;
+}
+
 int selmh_deck_forces(const selmh_deck *h, const double *x, double *f, double *energy) {
     if (!h || !x || !f) return 1;
     try {

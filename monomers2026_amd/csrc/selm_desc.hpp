@@ -47,6 +47,13 @@ struct RepDev {
     int *ctr;                  // [8]: overflow count per step parity [0,1], status flags [4]
     int *adj_ptr, *adj_i1, *adj_i2;   // CSR over beads: bonds touching each bead, bond order
     double *adj_K, *adj_r0;
+    int *ang_ptr, *ang_a, *ang_b, *ang_c;  // CSR over beads: harmonic angles (end, centre, end) touching each bead
+    double *ang_K, *ang_t0;
+    int lj_nt;                 // atom types of lj/cut (0: no pair term)
+    int *lj_type;              // [M] type of each bead (0-based)
+    double *lj_tab;            // [nt][nt][4]: eps, sigma, rc^2, 0
+    int *spec_ptr, *spec_j;    // CSR over beads: 1-2, 1-3, 1-4 partners whose lj weight is not 1
+    double *spec_w;
     // ---- scalars
     double dx, lo, hi, L;      // lo[0], hi[0] (reference wraps and grids with dim 0 only)
     double Lb[3];              // per-dimension box lengths (bond minimum image)

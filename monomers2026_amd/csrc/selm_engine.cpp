@@ -124,7 +124,9 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     return b;
 }
 
-static size_t bead_bytes(int M, int nadj, bool fp64) {
+// nadj: bond entries (2 per bond), nang: angle entries (3 per angle),
+// nt: lj/cut atom types, nspec: special-partner entries
+static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, int nspec = 0) {
     size_t b = 0;
     b += 7 * al(3 * (size_t)M * 8);
     b += al((size_t)M * 8);
@@ -132,6 +134,9 @@ static size_t bead_bytes(int M, int nadj, bool fp64) {
     b += al((size_t)kMaxTilesPerBead * M * 8);
     b += al(((size_t)M + 1) * 4);
     b += 2 * al((size_t)nadj * 4) + 2 * al((size_t)nadj * 8);
+    b += al(((size_t)M + 1) * 4) + 3 * al((size_t)nang * 4) + 2 * al((size_t)nang * 8);
+    b += al((size_t)M * 4) + al((size_t)nt * nt * 4 * 8);
+    b += al(((size_t)M + 1) * 4) + al((size_t)nspec * 4) + al((size_t)nspec * 8);
     return b;
 }
 
@@ -150,6 +155,19 @@ struct Replica {
     std::vector<double> hx, hv, hm, hfext;
     std::vector<int> adj_ptr, adj_i1, adj_i2;
     std::vector<double> adj_K, adj_r0;
+    std::vector<int> ang_ptr, ang_a, ang_b, ang_c;  // CSR entries per bead
+    std::vector<double> ang_K, ang_t0;
+    std::vector<int> ang_raw;                       // [na][3] as given
+    std::vector<double> ang_rawK, ang_rawT;
+    int lj_nt = 0;
+    std::vector<int> lj_type;
+    std::vector<double> lj_tab;
+    double special_lj[3] = {0, 0, 0};
+    std::vector<int> spec_ptr, spec_j;
+    std::vector<double> spec_w;
+    size_t bytes_beads() const {
+        return bead_bytes(M, (int)adj_i1.size(), fp64, (int)ang_a.size(), lj_nt, (int)spec_j.size());
+    }
     bool inited = false;
 };
 
@@ -216,6 +234,19 @@ static void carve_beads(Replica &R, char *base) {
     R.D.adj_i2 = (int *)take(nadj * 4);
     R.D.adj_K = (double *)take(nadj * 8);
     R.D.adj_r0 = (double *)take(nadj * 8);
+    const size_t nang = R.ang_a.size(), nspec = R.spec_j.size(), nt = (size_t)R.lj_nt;
+    R.D.ang_ptr = (int *)take(((size_t)M + 1) * 4);
+    R.D.ang_a = (int *)take(nang * 4);
+    R.D.ang_b = (int *)take(nang * 4);
+    R.D.ang_c = (int *)take(nang * 4);
+    R.D.ang_K = (double *)take(nang * 8);
+    R.D.ang_t0 = (double *)take(nang * 8);
+    R.D.lj_type = (int *)take((size_t)M * 4);
+    R.D.lj_tab = (double *)take(nt * nt * 4 * 8);
+    R.D.spec_ptr = (int *)take(((size_t)M + 1) * 4);
+    R.D.spec_j = (int *)take(nspec * 4);
+    R.D.spec_w = (double *)take(nspec * 8);
+    R.D.lj_nt = R.lj_nt;
 }
 
 static int replica_setup(Replica &R, const selm_params &p, bool fp64, std::string &err) {
@@ -324,6 +355,143 @@ static int host_set_beads(Replica &R, int M, const double *x, const double *v, c
     R.adj_K.clear();
     R.adj_r0.clear();
     R.nb = 0;
+    R.ang_raw.clear();
+    R.ang_rawK.clear();
+    R.ang_rawT.clear();
+    R.ang_ptr.assign((size_t)M + 1, 0);
+    R.ang_a.clear();
+    R.ang_b.clear();
+    R.ang_c.clear();
+    R.ang_K.clear();
+    R.ang_t0.clear();
+    R.lj_nt = 0;
+    R.lj_type.clear();
+    R.lj_tab.clear();
+    R.spec_ptr.assign((size_t)M + 1, 0);
+    R.spec_j.clear();
+    R.spec_w.clear();
+    return SELM_OK;
+}
+
+// lj/cut special-bond partners (LAMMPS special list from the bond graph):
+// 1-2 partners are bonded, 1-3 partners of those, 1-4 of those; a pair takes
+// the lowest order it has.  Only partners whose weight is not 1 are stored.
+static void build_special(Replica &R) {
+    R.spec_ptr.assign((size_t)R.M + 1, 0);
+    R.spec_j.clear();
+    R.spec_w.clear();
+    if (R.lj_nt == 0) return;
+    std::vector<std::vector<int>> nb(R.M);
+    for (int i = 0; i < R.M; i++)
+        for (int q = R.adj_ptr[i]; q < R.adj_ptr[i + 1]; q++) {
+            const int j = R.adj_i1[q] == i ? R.adj_i2[q] : R.adj_i1[q];
+            if (std::find(nb[i].begin(), nb[i].end(), j) == nb[i].end()) nb[i].push_back(j);
+        }
+    std::vector<int> order(R.M, 0);
+    for (int i = 0; i < R.M; i++) {
+        R.spec_ptr[i] = (int)R.spec_j.size();
+        std::vector<int> found;
+        std::vector<int> frontier{i};
+        order[i] = -1;
+        for (int o = 1; o <= 3; o++) {
+            std::vector<int> next;
+            for (int a : frontier)
+                for (int b : nb[a])
+                    if (order[b] == 0) {
+                        order[b] = o;
+                        found.push_back(b);
+                        next.push_back(b);
+                    }
+            frontier.swap(next);
+        }
+        std::sort(found.begin(), found.end());
+        for (int j : found) {
+            const double w = R.special_lj[order[j] - 1];
+            if (w != 1.0) {
+                R.spec_j.push_back(j);
+                R.spec_w.push_back(w);
+            }
+        }
+        for (int j : found) order[j] = 0;
+        order[i] = 0;
+    }
+    R.spec_ptr[R.M] = (int)R.spec_j.size();
+}
+
+static int host_set_angles(Replica &R, int na, const int *ijk, const double *K, const double *t0,
+                           std::string &err) {
+    if (na < 0 || (na > 0 && (!ijk || !K || !t0))) {
+        err = "selm_set_angles: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    std::vector<std::vector<int>> per(R.M);
+    for (int q = 0; q < na; q++) {
+        const int a = ijk[3 * q], b = ijk[3 * q + 1], c = ijk[3 * q + 2];
+        if (a < 0 || b < 0 || c < 0 || a >= R.M || b >= R.M || c >= R.M || a == b || b == c || a == c) {
+            err = "selm_set_angles: angle " + std::to_string(q) + " has invalid bead indices";
+            return SELM_ERR_ARG;
+        }
+        per[a].push_back(q);
+        per[b].push_back(q);
+        per[c].push_back(q);
+    }
+    R.ang_raw.assign(ijk, ijk + 3 * (size_t)na);
+    R.ang_rawK.assign(K, K + na);
+    R.ang_rawT.assign(t0, t0 + na);
+    R.ang_ptr.assign((size_t)R.M + 1, 0);
+    R.ang_a.clear();
+    R.ang_b.clear();
+    R.ang_c.clear();
+    R.ang_K.clear();
+    R.ang_t0.clear();
+    for (int i = 0; i < R.M; i++) {
+        R.ang_ptr[i] = (int)R.ang_a.size();
+        for (int q : per[i]) {  // ascending angle index
+            R.ang_a.push_back(ijk[3 * q]);
+            R.ang_b.push_back(ijk[3 * q + 1]);
+            R.ang_c.push_back(ijk[3 * q + 2]);
+            R.ang_K.push_back(K[q]);
+            R.ang_t0.push_back(t0[q]);
+        }
+    }
+    R.ang_ptr[R.M] = (int)R.ang_a.size();
+    return SELM_OK;
+}
+
+static int host_set_lj(Replica &R, int nt, const int *type, const double *eps, const double *sig, const double *rc,
+                       const double special[3], std::string &err) {
+    if (nt < 0 || (nt > 0 && (!type || !eps || !sig || !rc || !special))) {
+        err = "selm_set_pair_lj: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    if (nt == 0) {
+        R.lj_nt = 0;
+        R.lj_type.clear();
+        R.lj_tab.clear();
+        build_special(R);
+        return SELM_OK;
+    }
+    double Lmin = 1e300;
+    for (int a = 0; a < 3; a++) Lmin = std::min(Lmin, R.P.hi[a] - R.P.lo[a]);
+    for (int i = 0; i < R.M; i++)
+        if (type[i] < 0 || type[i] >= nt) {
+            err = "selm_set_pair_lj: bead type out of range";
+            return SELM_ERR_ARG;
+        }
+    R.lj_tab.assign((size_t)nt * nt * 4, 0.0);
+    for (int k = 0; k < nt * nt; k++) {
+        if (!(rc[k] >= 0) || 2 * rc[k] >= Lmin) {
+            err = "selm_set_pair_lj: cutoff must be >= 0 and below half the box edge";
+            return SELM_ERR_ARG;
+        }
+        R.lj_tab[4 * k] = eps[k];
+        R.lj_tab[4 * k + 1] = sig[k];
+        R.lj_tab[4 * k + 2] = rc[k] * rc[k];
+    }
+    R.lj_nt = nt;
+    R.lj_type.assign(type, type + R.M);
+    for (int o = 0; o < 3; o++) R.special_lj[o] = special[o];
+    build_special(R);
     return SELM_OK;
 }
 
@@ -359,6 +527,7 @@ static int host_set_bonds(Replica &R, int nb, const int *ij, const double *K, co
     }
     R.adj_ptr[R.M] = (int)R.adj_i1.size();
     R.nb = nb;
+    build_special(R);  // the special list follows the bond graph
     return SELM_OK;
 }
 
@@ -406,6 +575,25 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
         HIPCHK(hipMemcpyAsync(R.D.adj_i2, R.adj_i2.data(), na * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(R.D.adj_K, R.adj_K.data(), na * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(R.D.adj_r0, R.adj_r0.data(), na * 8, hipMemcpyHostToDevice, st));
+    }
+    if (R.ang_ptr.size() == (size_t)R.M + 1)
+        HIPCHK(hipMemcpyAsync(R.D.ang_ptr, R.ang_ptr.data(), R.ang_ptr.size() * 4, hipMemcpyHostToDevice, st));
+    if (!R.ang_a.empty()) {
+        const size_t n = R.ang_a.size();
+        HIPCHK(hipMemcpyAsync(R.D.ang_a, R.ang_a.data(), n * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.ang_b, R.ang_b.data(), n * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.ang_c, R.ang_c.data(), n * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.ang_K, R.ang_K.data(), n * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.ang_t0, R.ang_t0.data(), n * 8, hipMemcpyHostToDevice, st));
+    }
+    if (R.lj_nt > 0) {
+        HIPCHK(hipMemcpyAsync(R.D.lj_type, R.lj_type.data(), (size_t)R.M * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.lj_tab, R.lj_tab.data(), R.lj_tab.size() * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(R.D.spec_ptr, R.spec_ptr.data(), R.spec_ptr.size() * 4, hipMemcpyHostToDevice, st));
+        if (!R.spec_j.empty()) {
+            HIPCHK(hipMemcpyAsync(R.D.spec_j, R.spec_j.data(), R.spec_j.size() * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(R.D.spec_w, R.spec_w.data(), R.spec_w.size() * 8, hipMemcpyHostToDevice, st));
+        }
     }
     const size_t s = R.fp64 ? 8 : 4;
     HIPCHK(hipMemsetAsync(R.D.u, 0, 3 * (size_t)R.G.N * s, st));
@@ -739,7 +927,7 @@ static int ctx_alloc_beads(selm_ctx *c) {
         (void)hipFree(c->bead_mem);
         c->bead_mem = nullptr;
     }
-    size_t bb = bead_bytes(c->rep.M, (int)c->rep.adj_i1.size(), c->rep.fp64);
+    size_t bb = c->rep.bytes_beads();
     c->rep.bead_sz = bb;
     if (hipMalloc(&c->bead_mem, std::max<size_t>(bb, 256)) != hipSuccess) {
         err = "hipMalloc of bead buffers failed";
@@ -762,6 +950,21 @@ int selm_set_bonds(selm_ctx *c, int nb, const int *ij, const double *K, const do
     CTX_CHECK(c);
     int rc = host_set_bonds(c->rep, nb, ij, K, r0, c->err);
     if (rc) return rc;
+    return ctx_alloc_beads(c);
+}
+
+int selm_set_angles(selm_ctx *c, int na, const int *ijk, const double *K, const double *theta0) {
+    CTX_CHECK(c);
+    int rc = host_set_angles(c->rep, na, ijk, K, theta0, c->err);
+    if (rc) return rc;
+    return ctx_alloc_beads(c);
+}
+
+int selm_set_pair_lj(selm_ctx *c, int ntypes, const int *type, const double *eps, const double *sigma,
+                     const double *rc, const double special_lj[3]) {
+    CTX_CHECK(c);
+    int r = host_set_lj(c->rep, ntypes, type, eps, sigma, rc, special_lj, c->err);
+    if (r) return r;
     return ctx_alloc_beads(c);
 }
 

@@ -16,8 +16,9 @@
 //                     only; the compile-time-length X pass computes u* itself)
 //   X, Y, Z+projection, Y, X passes of the 3-D FFT (selm_fft_kernels.hip for
 //                     the compile-time lengths, below for the others)
-//   k_bond_forces     thread per bead: harmonic bonds + external force at
-//                     x^{n+1}; advances each replica's step counter
+//   k_bond_forces     thread per bead: harmonic bonds and angles, lj/cut pairs
+//                     and the external force at x^{n+1}; advances each
+//                     replica's step counter
 #include "selm_fluid.hpp"
 #include "selm_kernels.hpp"
 
@@ -596,6 +597,78 @@ __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ 
             f0 -= del[0] * fbond;
             f1 -= del[1] * fbond;
             f2 -= del[2] * fbond;
+        }
+    }
+    // harmonic angles (LAMMPS angle_style harmonic): E = K (theta - theta0)^2
+    // at the centre bead; -dE/dx = (2 K dtheta / sin theta) dcos/dx, with sin
+    // theta held away from 0 by 0.001 as LAMMPS does.
+    for (int q = R.ang_ptr[b]; q < R.ang_ptr[b + 1]; q++) {
+        const int ia = R.ang_a[q], ib = R.ang_b[q], ic = R.ang_c[q];
+        double u[3], w[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double du = R.x[3 * ia + d] - R.x[3 * ib + d];
+            if (fabs(du) > 0.5 * R.Lb[d]) du += (du < 0.0) ? R.Lb[d] : -R.Lb[d];
+            double dw = R.x[3 * ic + d] - R.x[3 * ib + d];
+            if (fabs(dw) > 0.5 * R.Lb[d]) dw += (dw < 0.0) ? R.Lb[d] : -R.Lb[d];
+            u[d] = du;
+            w[d] = dw;
+        }
+        const double ru2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double rw2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        const double ru = sqrt(ru2), rw = sqrt(rw2);
+        double cs = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
+        cs = fmin(1.0, fmax(-1.0, cs));
+        double sn = sqrt(1.0 - cs * cs);
+        if (sn < 0.001) sn = 0.001;
+        const double dth = acos(cs) - R.ang_t0[q];
+        const double g = 2.0 * R.ang_K[q] * dth / sn;
+        double fa[3], fc[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            fa[d] = g * (w[d] / (ru * rw) - cs * u[d] / ru2);
+            fc[d] = g * (u[d] / (ru * rw) - cs * w[d] / rw2);
+        }
+        if (b == ia) {
+            f0 += fa[0];
+            f1 += fa[1];
+            f2 += fa[2];
+        } else if (b == ic) {
+            f0 += fc[0];
+            f1 += fc[1];
+            f2 += fc[2];
+        } else {
+            f0 -= fa[0] + fc[0];
+            f1 -= fa[1] + fc[1];
+            f2 -= fa[2] + fc[2];
+        }
+    }
+    // lj/cut (LAMMPS pair_style lj/cut): E = 4 eps ((s/r)^12 - (s/r)^6), r < rc,
+    // minimum image; 1-2/1-3/1-4 partners carry the special weight.  Every
+    // other bead is visited in ascending index.
+    if (R.lj_nt > 0) {
+        const int ti = R.lj_type[b];
+        const int s0 = R.spec_ptr[b], s1 = R.spec_ptr[b + 1];
+        const double xb = R.x[3 * b], yb = R.x[3 * b + 1], zb = R.x[3 * b + 2];
+        for (int j = 0; j < R.M; j++) {
+            if (j == b) continue;
+            double del[3] = {xb - R.x[3 * j], yb - R.x[3 * j + 1], zb - R.x[3 * j + 2]};
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                if (fabs(del[d]) > 0.5 * R.Lb[d]) del[d] += (del[d] < 0.0) ? R.Lb[d] : -R.Lb[d];
+            const double r2 = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+            const double *tab = R.lj_tab + 4 * (ti * R.lj_nt + R.lj_type[j]);
+            if (r2 >= tab[2]) continue;
+            double wgt = 1.0;
+            for (int q = s0; q < s1; q++)
+                if (R.spec_j[q] == j) wgt = R.spec_w[q];
+            if (wgt == 0.0) continue;
+            const double s2 = tab[1] * tab[1] / r2;
+            const double s6 = s2 * s2 * s2;
+            const double fr = wgt * 24.0 * tab[0] * (2.0 * s6 * s6 - s6) / r2;
+            f0 += fr * del[0];
+            f1 += fr * del[1];
+            f2 += fr * del[2];
         }
     }
     R.fc[3 * b + 0] = f0 + R.fext[3 * b + 0];

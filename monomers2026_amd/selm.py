@@ -37,6 +37,7 @@ EXPORTS = [
     "selm_pool_create", "selm_pool_destroy", "selm_pool_last_error", "selm_pool_admit",
     "selm_pool_step", "selm_pool_sync", "selm_pool_get_beads", "selm_pool_get_field",
     "selm_pool_timestep", "selm_pool_retire", "selm_pool_stats", "selm_pool_stream",
+    "selm_msd", "selm_set_angles", "selm_set_pair_lj",
 ]
 
 
@@ -116,6 +117,9 @@ def load(path=None):
     L.selm_pool_stats.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), ip]
     L.selm_pool_stream.argtypes = [vp]
     L.selm_pool_stream.restype = vp
+    L.selm_msd.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, dp, C.c_int, C.c_int, dp]
+    L.selm_set_angles.argtypes = [vp, C.c_int, ip, dp, dp]
+    L.selm_set_pair_lj.argtypes = [vp, C.c_int, ip, dp, dp, dp, dp]
     _lib = L
     return L
 
@@ -198,6 +202,22 @@ class System:
         K = f64(K, (-1,))
         r0 = f64(r0, (-1,))
         self._chk(self.L.selm_set_bonds(self.h, ij.shape[0], _i(ij), _d(K), _d(r0)), "set_bonds")
+
+    def set_angles(self, ijk, K, theta0):
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        K = f64(K, (-1,))
+        t0 = f64(theta0, (-1,))
+        self._chk(self.L.selm_set_angles(self.h, ijk.shape[0], _i(ijk), _d(K), _d(t0)), "set_angles")
+
+    def set_pair_lj(self, type_, eps, sigma, rc, special_lj=(0.0, 0.0, 0.0)):
+        t = np.ascontiguousarray(type_, dtype=np.int32).reshape(-1)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        nt = eps.shape[0]
+        sigma = f64(sigma, (nt, nt))
+        rc = f64(rc, (nt, nt))
+        sp = f64(special_lj, (3,))
+        self._chk(self.L.selm_set_pair_lj(self.h, nt, _i(t), _d(f64(eps, (nt, nt))), _d(sigma), _d(rc), _d(sp)),
+                  "set_pair_lj")
 
     def set_external_force(self, idx, F):
         idx = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1)

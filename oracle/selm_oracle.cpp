@@ -665,4 +665,40 @@ void oracle_normal_pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t 
 
 void oracle_bond_forces(void *op, const double *x, double *f) { bond_forces(*(Oracle *)op, x, f); }
 
+// Mean-square displacement of a trajectory, py/Diffusion Coefficient/MSD.py:15-69:
+// frames [nframes][3][natoms] float; origins from frame floor(nframes/5);
+// lag = dot*mult; per dimension the lag displacement D (FP32) goes to its
+// minimum image with X = box[0]/2 (the script's x half-box, used for every
+// dimension): (D-2X)^2 if D >= X, (D+2X)^2 if D <= -X, else D^2, each squared
+// in FP32; summed in FP64 over beads, origins and dimensions (in that loop
+// order); MSD = sum / ((F-lag) * natoms), NaN when no origin is left.
+void oracle_msd(const float *frames, int nframes, int natoms, const double box[3], int nlags, int mult,
+                double *msd) {
+    const int start = nframes / 5;
+    const int F = nframes - start;
+    const float X = (float)box[0] / 2.0f;
+    msd[0] = 0.0;
+    for (int dot = 1; dot < nlags; dot++) {
+        const int lag = dot * mult;
+        const long long Fd = (long long)F - lag;
+        if (Fd <= 0) {
+            msd[dot] = NAN;
+            continue;
+        }
+        double acc = 0.0;
+        for (int a = 0; a < natoms; a++)
+            for (long long n = 0; n < Fd; n++)
+                for (int d = 0; d < 3; d++) {
+                    const float x1 = frames[((size_t)(start + n + lag) * 3 + d) * natoms + a];
+                    const float x0 = frames[((size_t)(start + n) * 3 + d) * natoms + a];
+                    float D = x1 - x0;
+                    if (D >= X) D = D - 2.0f * X;
+                    else if (D <= -X) D = D + 2.0f * X;
+                    const float sq = D * D;
+                    acc += (double)sq;
+                }
+        msd[dot] = acc / ((double)Fd * natoms);
+    }
+}
+
 }  // extern "C"

@@ -66,6 +66,8 @@ void oracle_normal_pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t 
                         uint32_t slot, double out[2]);
 /* harmonic bonds + constant external force at the given positions */
 void oracle_bond_forces(void *o, const double *x, double *f);
+void oracle_msd(const float *frames, int nframes, int natoms, const double box[3], int nlags, int mult,
+                double *msd);
 
 #ifdef __cplusplus
 }

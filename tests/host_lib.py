@@ -39,6 +39,8 @@ def lib():
         L.selmh_deck_bonds.argtypes = [C.c_void_p, ip, dp, dp]
         L.selmh_deck_external_force.argtypes = [C.c_void_p, dp]
         L.selmh_deck_forces.argtypes = [C.c_void_p, dp, dp, dp]
+        L.selmh_deck_angles.argtypes = [C.c_void_p, ip, dp, dp]
+        L.selmh_deck_pair_lj.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.selmh_dcd_open.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]
         L.selmh_dcd_frame.argtypes = [C.c_void_p, C.c_longlong, dp, dp]
         L.selmh_dcd_close.argtypes = [C.c_void_p]
@@ -83,6 +85,13 @@ class Deck:
         lib().selmh_deck_bonds(self.h, _i(self.ij), _d(self.K), _d(self.r0))
         self.fext = np.zeros((self.n, 3))
         lib().selmh_deck_external_force(self.h, _d(self.fext))
+        self.ijk = np.zeros((self.na, 3), np.int32); self.aK = np.zeros(self.na); self.at0 = np.zeros(self.na)
+        lib().selmh_deck_angles(self.h, _i(self.ijk), _d(self.aK), _d(self.at0))
+        nt = self.nt
+        self.lj = None
+        eps, sig, rc, sp = np.zeros((nt, nt)), np.zeros((nt, nt)), np.zeros((nt, nt)), np.zeros(3)
+        if lib().selmh_deck_pair_lj(self.h, _d(eps), _d(sig), _d(rc), _d(sp)) == 0:
+            self.lj = (eps, sig, rc, sp)
 
     def forces(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -139,6 +139,19 @@ def set_threads(n):
     lib().oracle_set_threads(int(n))
 
 
+def msd(frames, box_x, nlags, mult=1):
+    """oracle_msd: frames [F][3][natoms] float32 -> MSD[0..nlags-1]."""
+    frames = np.ascontiguousarray(frames, dtype=np.float32)
+    F, _, n = frames.shape
+    out = np.zeros(nlags)
+    box = np.array([box_x, box_x, box_x], dtype=np.float64)
+    L = lib()
+    L.oracle_msd.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int,
+                             C.POINTER(C.c_double)]
+    L.oracle_msd(frames.ctypes.data_as(C.POINTER(C.c_float)), F, n, _dp(box), int(nlags), int(mult), _dp(out))
+    return out
+
+
 class Oracle:
     """One SELM system (Eulerian_Period + bead-spring beads) on the CPU, FP64."""
 

@@ -24,10 +24,15 @@ def _blocks(path, n):
     return a
 
 
-def test_driver_runs_the_small_deck(gpu_lib, tmp_path):
+@pytest.mark.parametrize("host_forces", [0, 1])
+def test_driver_runs_the_small_deck(gpu_lib, tmp_path, host_forces):
+    """host_forces 0: angles and lj/cut on the device (default); 1: on the
+    host, handed over every step through selm_set_conservative_force."""
     import oracle_lib as O
     run = tmp_path / "deck"
     shutil.copytree(H.DECK_SMALL, run)
+    with open(run / "Model.SELM", "a") as f:
+        f.write("SELM_HostForces %d\n" % host_forces)
     assert os.path.exists(H.DRIVER), "selm_hip_driver is not built"
     p = subprocess.run([H.DRIVER, str(run)], capture_output=True, text=True, timeout=120)
     print(p.stdout[-400:], p.stderr[-400:], flush=True)
@@ -63,7 +68,7 @@ def test_driver_runs_the_small_deck(gpu_lib, tmp_path):
         x, v, ft = ref[s]
         for name, got, want in (("x", X[k], x), ("v", V[k], v), ("f", F[k], ft)):
             err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-12)
-            print("driver vs oracle step %d %s: %.2e" % (s, name, err), flush=True)
+            print("driver (host forces %d) vs oracle step %d %s: %.2e" % (host_forces, s, name, err), flush=True)
             assert err < 2e-5, (s, name, err)
 
     icntrl, titles, nat, frames = H.read_dcd(str(run / "Model.dcd"))
