@@ -174,42 +174,143 @@ __global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__re
 // ascending bead index, so the result does not depend on scheduling or on the
 // other replicas of a batch.
 // ===========================================================================
-template <typename T>
-struct Acc;
-template <>
-struct Acc<double> {  // plain FP64 sum
-    static __device__ void add(double *s, double *, double x) { *s += x; }
-    static __device__ double value(const double *s, const double *) { return *s; }
-};
-template <>
-struct Acc<float> {  // Neumaier-compensated FP32 sum
-    static __device__ void add(float *s, float *c, float x) {
-        float a = *s, t = a + x;
-        if (fabsf(a) >= fabsf(x)) *c += (a - t) + x;
-        else *c += (x - t) + a;
-        *s = t;
-    }
-    static __device__ float value(const float *s, const float *c) { return *s + *c; }
-};
-
+// The cell sums are FP64 in both precisions: each product -t_f * w is formed
+// in T and added to an FP64 accumulator in LDS, and the tile is rounded to T
+// once when it is written.
 constexpr int kTileCells = 512;  // tile edge <= 8
 constexpr int kSpreadChunk = 32;
+constexpr int kSpreadBatch = 8;  // beads whose (cell, value) a lane holds before adding
 constexpr int kSpreadThreads = 256;
 constexpr int kListCap = 2048;
+
+// Records of chunk `base` (nq >= 1 beads of the sorted list) into LDS, by NT
+// threads (lt = 0 .. NT-1) in 16-byte pieces.  Every thread issues all of its
+// loads (indices clamped into the chunk) before its first store, so a chunk
+// costs about one global-load latency.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <typename T, int NT>
+__device__ __forceinline__ void spread_stage(SpreadRec<T> *dst, const SpreadRec<T> *__restrict__ brec,
+                                             const int *sorted, int base, int nq, int lt) {
+    static_assert(sizeof(SpreadRec<T>) % 16 == 0, "records are whole 16-byte pieces");
+    constexpr int V = sizeof(SpreadRec<T>) / 16;
+    constexpr int PER = (kSpreadChunk * V + NT - 1) / NT;
+    const int tot = nq * V;
+    u32x4 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = min(lt + q * NT, tot - 1);
+        const int k = idx / V;
+        v[q] = ((const u32x4 *)&brec[sorted[base + k]])[idx - k * V];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = lt + q * NT;
+        if (idx < tot) {
+            const int k = idx / V;
+            ((u32x4 *)&dst[k])[idx - k * V] = v[q];
+        }
+    }
+}
+
+struct SpreadTile {
+    int nx, ny, nz;  // grid
+    int ox, oy, oz;  // tile origin
+    int ex, ey, ez;  // tile extent (ragged at the high end)
+    int te0, te1;    // tile edge (cell index strides)
+};
+
+// v of lane k (k uniform), as a scalar
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int k) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
+// Wave d adds component d of the chunk's nq beads, in list order, lane =
+// stencil point (li, lj, lk).  A lane's cell on an axis is (p + l) mod n - o;
+// it belongs to the tile if that lies in [0, e).
+//
+// FAST: the offset q = p - o, taken as q - n when q >= e, is the same for the
+// whole wave (a scalar), and each lane only adds l.  This gives the cell
+// above whenever n >= te + 3 on every axis (so n > 8 >= 5):
+// - p in [o, o + e): a wrapped lane (p + l >= n) cannot come back into the
+//   tile, since p + l - n < o + e + 3 - n <= o;
+// - p >= o + e: the stencil meets the tile only through the wrap, where the
+//   cell is p + l - n - o = (q - n) + l;
+// - p < o: a wrapped cell (<= 2) would need o <= 2, while wrapping needs
+//   p >= n - 3, so o > n - 3 > 2.
+template <typename T, bool FAST>
+__device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T> *Q, int nq, int d, int lane,
+                                                 const SpreadTile &g) {
+    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
+    const int lofs = li + g.te0 * (lj + g.te1 * lk);
+    // lane m (< 32) holds bead m's stencil base and force for this component;
+    // the loop takes them per bead with readlane
+    static_assert(kSpreadChunk <= 32, "one bead per lane of the half-wave");
+    const SpreadRec<T> &M = Q[min(lane & 31, nq - 1)];
+    const int px = M.p[d][0], py = M.p[d][1], pz = M.p[d][2];
+    const T fm = M.f[d];
+    for (int k0 = 0; k0 < nq; k0 += kSpreadBatch) {
+        int cell[kSpreadBatch];
+        double val[kSpreadBatch];
+#pragma unroll
+        for (int j = 0; j < kSpreadBatch; j++) {
+            const int k = k0 + j;  // < kSpreadChunk; beads past nq are masked below
+            const SpreadRec<T> &B = Q[min(k, nq - 1)];
+            const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
+            const T f = readlane_t(fm, k);
+            bool in;
+            int c;
+            if constexpr (FAST) {
+                int qx = __builtin_amdgcn_readlane(px, k) - g.ox;
+                int qy = __builtin_amdgcn_readlane(py, k) - g.oy;
+                int qz = __builtin_amdgcn_readlane(pz, k) - g.oz;
+                qx -= qx >= g.ex ? g.nx : 0;
+                qy -= qy >= g.ey ? g.ny : 0;
+                qz -= qz >= g.ez ? g.nz : 0;
+                in = (unsigned)(qx + li) < (unsigned)g.ex && (unsigned)(qy + lj) < (unsigned)g.ey &&
+                     (unsigned)(qz + lk) < (unsigned)g.ez;
+                c = __builtin_amdgcn_readfirstlane(d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx) + lofs;
+            } else {
+                int cx = __builtin_amdgcn_readlane(px, k) + li;
+                cx -= cx >= g.nx ? g.nx : 0;
+                int cy = __builtin_amdgcn_readlane(py, k) + lj;
+                cy -= cy >= g.ny ? g.ny : 0;
+                int cz = __builtin_amdgcn_readlane(pz, k) + lk;
+                cz -= cz >= g.nz ? g.nz : 0;
+                cx -= g.ox;
+                cy -= g.oy;
+                cz -= g.oz;
+                in = (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
+                c = d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx;
+            }
+            cell[j] = in && k < nq ? c : -1;
+            val[j] = (double)(f * w);
+        }
+        // in bead order: a cell met by several beads of the batch is added
+        // to in the order of the sorted list
+#pragma unroll
+        for (int j = 0; j < kSpreadBatch; j++)
+            if (cell[j] >= 0) acc[cell[j]] += val[j];
+    }
+}
 
 template <typename T>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps, int nrep,
                                                            const long long *__restrict__ start,
                                                            long long total) {
-    constexpr int ACCW = sizeof(T) == 4 ? 2 * 3 * kTileCells : 3 * kTileCells;  // sum (+ compensation)
-    static_assert(ACCW * sizeof(T) >= kListCap * sizeof(int), "list shares the accumulator space");
+    static_assert(3 * kTileCells * sizeof(double) >= kListCap * sizeof(int), "list shares the accumulators");
     __shared__ int sorted[kListCap];
-    __shared__ T accbuf[ACCW];  // holds the unsorted list first, then the accumulators
+    __shared__ double acc[3 * kTileCells];  // holds the unsorted list first, then the cell sums
     __shared__ int nlist;
-    __shared__ SpreadRec<T> rec[kSpreadChunk];
-    int *list = (int *)accbuf;
-    T *acc_s = accbuf;
-    T *acc_c = accbuf + (sizeof(T) == 4 ? 3 * kTileCells : 0);
+    __shared__ __align__(16) SpreadRec<T> rec[2][kSpreadChunk];  // chunk c in rec[c & 1]
+    int *list = (int *)acc;
 
     const long long g = blockIdx.x;
     if (g >= total) return;
@@ -252,48 +353,34 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
         for (int q = 0; q < n; q++) rank += (list[q] < bi) ? 1 : 0;
         sorted[rank] = bi;
     }
-    __syncthreads();
-    for (int c = tid; c < ACCW; c += kSpreadThreads) accbuf[c] = (T)0;
-
+    __syncthreads();  // sorted is complete and the list is read: its space becomes the accumulators
     const SpreadRec<T> *brec = (const SpreadRec<T> *)R.brec;
-    constexpr int W = sizeof(SpreadRec<T>) / 4;
-    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
-    const int te0 = R.te[0], te1 = R.te[1];
-    for (int base = 0; base < n; base += kSpreadChunk) {
-        const int nq = min(kSpreadChunk, n - base);
-        __syncthreads();  // accumulators cleared / previous chunk consumed
-        for (int idx = tid; idx < nq * W; idx += kSpreadThreads) {
-            const int k = idx / W, wd = idx - k * W;
-            ((uint32_t *)&rec[k])[wd] = ((const uint32_t *)&brec[sorted[base + k]])[wd];
-        }
-        __syncthreads();
-        if (wv < 3) {
-            const int d = wv;
-            for (int k = 0; k < nq; k++) {
-                const SpreadRec<T> &B = rec[k];
-                int cx = B.p[d][0] + li;
-                cx -= cx >= nx ? nx : 0;
-                int cy = B.p[d][1] + lj;
-                cy -= cy >= ny ? ny : 0;
-                int cz = B.p[d][2] + lk;
-                cz -= cz >= nz ? nz : 0;
-                cx -= ox;
-                cy -= oy;
-                cz -= oz;
-                if ((unsigned)cx < (unsigned)ex && (unsigned)cy < (unsigned)ey && (unsigned)cz < (unsigned)ez) {
-                    const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
-                    const int cell = d * kTileCells + (cz * te1 + cy) * te0 + cx;
-                    Acc<T>::add(&acc_s[cell], &acc_c[sizeof(T) == 4 ? cell : 0], B.f[d] * w);
-                }
-            }
-        }
-    }
+    spread_stage<T, kSpreadThreads>(rec[0], brec, sorted, 0, min(kSpreadChunk, n), tid);
+    for (int c = tid; c < 3 * kTileCells; c += kSpreadThreads) acc[c] = 0.0;
     __syncthreads();
+
+    // Waves 0..2 add chunk c (wave d: component d, lane: stencil point) while
+    // wave 3 stages chunk c+1 into the other buffer.
+    const int te0 = R.te[0], te1 = R.te[1];
+    // no stencil wraps back into the tile it starts in (see spread_add_chunk)
+    const bool fast = nx >= te0 + 3 && ny >= te1 + 3 && nz >= R.te[2] + 3;
+    for (int c = 0, base = 0; base < n; c++, base += kSpreadChunk) {
+        const int nq = min(kSpreadChunk, n - base);
+        if (wv == 3) {
+            const int nb2 = base + kSpreadChunk;
+            if (nb2 < n) spread_stage<T, 64>(rec[(c + 1) & 1], brec, sorted, nb2, min(kSpreadChunk, n - nb2), lane);
+        } else {
+            const SpreadTile tl{nx, ny, nz, ox, oy, oz, ex, ey, ez, te0, te1};
+            if (fast) spread_add_chunk<T, true>(acc, rec[c & 1], nq, wv, lane, tl);
+            else spread_add_chunk<T, false>(acc, rec[c & 1], nq, wv, lane, tl);
+        }
+        __syncthreads();  // chunk c consumed, chunk c+1 staged
+    }
     const int tcells = te0 * te1 * R.te[2];
     T *pool = (T *)R.pool + (long long)t * 3 * tcells;
     for (int c = tid; c < 3 * tcells; c += kSpreadThreads) {
         const int d = c / tcells, cell = d * kTileCells + (c - d * tcells);
-        pool[c] = Acc<T>::value(&acc_s[cell], &acc_c[sizeof(T) == 4 ? cell : 0]);
+        pool[c] = (T)acc[cell];
     }
     if (tid == 0) R.tstamp[t] = (int)step;
 }

@@ -71,13 +71,17 @@ def test_chains1000_64_fp64(gpu_lib, noise):
         _compare(sysg, o, FP64_TOL, "chains1000_64 noise=%d step=%d" % (noise, o.timestep), box=64.0)
 
 
-@pytest.mark.parametrize("dims", [(30, 24, 20), (36, 25, 18)])
+@pytest.mark.parametrize("dims", [(30, 24, 20), (36, 25, 18), (10, 9, 8), (12, 15, 20)])
 def test_random_fluid_one_step_fp64(gpu_lib, dims):
     """Steps from a random (not divergence-free) u^n on non-cubic mixed-radix
     grids: gather, spread, u*, the FFT radices and the projection against the
     oracle's real-space stencils.  (30, 24, 20) runs the compile-time-length
     FFT kernels; (36, 25, 18) has no specialised length and runs the generic
-    kernels, with ragged 7-cell tiles along y."""
+    kernels, with ragged 7-cell tiles along y.  The spread takes its per-lane
+    wrap path on (10, 9, 8), where a stencil can wrap back into the tile it
+    starts in (one 8-cell tile along z, ragged 2-cell last tiles along x
+    and y), and its scalar-offset path on (12, 15, 20), with ragged 4-, 7-
+    and 4-cell last tiles."""
     import oracle_lib as O
     from monomers2026_amd import selm
     rng = np.random.default_rng(5)
