@@ -83,15 +83,15 @@ def stage_bytes(w, s):
 # for a copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the
 # per-kernel table is profiles/pmc_traffic_dilute20k_128.md.
 PMC_TRAFFIC_FP32 = {
-    "thermal_noise": 25827314,
-    "bead_update": 33745575,
-    "spread": 17959920,
-    "ustar_fft_x": 63794839,
-    "fft_y": 51308419,
-    "fft_z_projection": 76951723,
-    "ifft_y": 51303299,
-    "ifft_x": 50879236,
-    "bond_forces": 2584375,
+    "thermal_noise": 25835506,
+    "bead_update": 33745288,
+    "spread": 18019950,
+    "ustar_fft_x": 63901715,
+    "fft_y": 51347713,
+    "fft_z_projection": 77068966,
+    "ifft_y": 51335938,
+    "ifft_x": 50921219,
+    "bond_forces": 2709425,
 }
 COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
 
