@@ -77,6 +77,22 @@ def stage_bytes(w, s):
     }
 
 
+def reference_stage_bytes(w, s):
+    """Bytes per step of the reference's own stage decomposition, each stage
+    reading its inputs once and writing its outputs once (cuda/SELM.cu:73-90
+    with cuda/SELM_Eulerian_Period.cu:96-153): u* 9Ns, div 4Ns, r2c Ns+2sNc,
+    the Poisson multiplier 4sNc, c2r Ns+2sNc, grad 7Ns, mean 9Ns, thermal
+    field 6sNc + 3(2sNc + Ns) + 6Ns, interpolation M(24 + 192s), spread
+    M(24 + 3s) + 192Ms, bead state 152M.  A fused implementation moves fewer
+    bytes; this is the yardstick the reference's structure sets."""
+    N = w.edge ** 3
+    Nc = (w.edge // 2 + 1) * w.edge * w.edge
+    M = w.M
+    grid = (9 + 4 + 1 + 1 + 7 + 9 + 3 + 6) * N * s + (2 + 4 + 2 + 6 + 6) * s * Nc
+    beads = M * (24 + 192 * s) + M * (24 + 3 * s) + 192 * M * s + 152 * M
+    return grid + beads
+
+
 # HBM bytes per launch of each stage's kernel at dilute20k_128 in FP32-bulk,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this script
 # (--fp32-only --copy-calibration 256), each scaled by what the counter reads
@@ -293,6 +309,7 @@ def main():
         classes[c] = {"algorithmic_bytes": by, "ms": round(ms, 5), "GBs": round(gbs, 1),
                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
     step_bytes = sum(b32.values())
+    ref_bytes = reference_stage_bytes(w, 4)
     out = {
         "metric": "bead-steps/s at 128^3 FP32-bulk (GPU), SELM step, 20k beads",
         "value": round(bead_steps32, 1),
@@ -317,6 +334,9 @@ def main():
         "stage_ms_fp64": {k: round(v, 5) for k, v in avg64.items()},
         "class_roofline_fp32": classes,
         "step_algorithmic_GBs_fp32": round(step_bytes / (per32 * 1e-3) / 1e9, 1),
+        "step_reference_stages_fp32": {"bytes": ref_bytes,
+                                       "GBs": round(ref_bytes / (per32 * 1e-3) / 1e9, 1),
+                                       "frac": round(ref_bytes / (per32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roof,
     }
     if not args.no_ensemble:
