@@ -83,6 +83,17 @@ __device__ __forceinline__ void copy_batched(const LD &ld, const ST &st) {
 // ------------------------------------------------------------------ Xf
 // Workgroup: component d, plane z, rows y0 .. y0+RB-1 of u*; rows are paired
 // into complex sequences (even row = real part, odd row = imaginary part).
+//
+// u* (the arithmetic of ustar_at, cuda/SELM_Kernel.cu:155-195) is formed on
+// load.  The rows y0-1 .. y0+RB of plane z are staged in LDS first, so the
+// x and y neighbours come from LDS; the z neighbours and the spread pool are
+// read per element, a batch of elements at a time, with the tiles' stamps
+// read into LDS beforehand so no load waits on another.
+template <typename T>
+struct XfLoads {
+    T zp, zm, f;  // u at z+1 and z-1, the tile's spread value
+};
+
 template <typename T, int N>
 __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C> *tw,
                                         cplx<typename FastCfg<T>::L> *a) {
@@ -95,18 +106,67 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
     const int z = (int)(dz % nz), d = (int)(dz / nz);
     const int y0 = yb * RB;
     const int step = (int)(R.ts[0] + 1);
-    copy_batched<NT, RB * N, 4>(
+    const long long Nyx = (long long)N * ny;
+    const T *ud = (const T *)R.u + d * R.N;
+    T *X = (T *)(a + S * N);  // rows y0-1 .. y0+RB of plane z (periodic in y)
+
+    // stamps of the tiles under rows y0 .. yl: tile edges are >= 4 cells
+    constexpr int FW = N / 4 + 2, FH = RB / 4 + 2;
+    __shared__ int okf[FH * FW];
+    const int te0 = R.te[0], te1 = R.te[1], te2 = R.te[2], tn0 = R.tn[0], tn1 = R.tn[1];
+    const int tz = z / te2;
+    const int yl = min(y0 + RB, ny) - 1;
+    const int ty0 = y0 / te1, nty = yl / te1 - ty0 + 1;
+    for (int i = threadIdx.x; i < nty * tn0; i += NT) {
+        const int q = i / tn0, tx = i - q * tn0;
+        okf[q * FW + tx] = R.tstamp[(tz * tn1 + ty0 + q) * tn0 + tx] == step;
+    }
+    constexpr int SROWS = (RB + 2) * N;
+    copy_batched<NT, SROWS, (SROWS + NT - 1) / NT>(
+        [&](int idx) {
+            const int r = idx / N, e = idx - r * N;
+            int y = y0 - 1 + r;
+            y += y < 0 ? ny : 0;
+            y -= y >= ny ? ny : 0;
+            return ud[(long long)z * Nyx + (long long)y * N + e];
+        },
+        [&](int idx, T v) { X[idx] = v; });
+    __syncthreads();
+
+    const T *up = ud + (long long)(z + 1 < nz ? z + 1 : 0) * Nyx;
+    const T *um = ud + (long long)(z > 0 ? z - 1 : nz - 1) * Nyx;
+    const T *pool = (const T *)R.pool;
+    const int tcells = te0 * te1 * te2;
+    const C dxsq = (C)(R.dx * R.dx), mu = (C)R.mu, dt = (C)R.dt, rho = (C)R.rho;
+    copy_batched<NT, RB * N, 8>(
         [&](int idx) {
             const int rl = idx / N, e = idx - rl * N;
             const int y = min(y0 + rl, ny - 1);  // rows past the grid are loaded, then zeroed
-            int tyz, lyz;
-            spread_row(R, y, z, tyz, lyz);
-            const C v = ustar_at<T, C>(R, d, e, y, z, tyz, lyz, step);
-            return y0 + rl < ny ? v : (C)0;
+            const int ty = y / te1, tx = e / te0;
+            const long long t = (long long)(tz * tn1 + ty) * tn0 + tx;
+            const int lyz = ((z - tz * te2) * te1 + (y - ty * te1)) * te0;
+            const long long row = (long long)y * N + e;
+            XfLoads<T> v;
+            v.zp = up[row];
+            v.zm = um[row];
+            v.f = pool[(t * 3 + d) * tcells + lyz + (e - tx * te0)];
+            return v;
         },
-        [&](int idx, C v) {
+        [&](int idx, XfLoads<T> v) {
             const int rl = idx / N, e = idx - rl * N;
-            ((L *)&a[e * S + (rl >> 1)])[rl & 1] = (L)v;
+            const int y = y0 + rl;
+            C us = (C)0;
+            if (y < ny) {
+                // neighbour order x+, x-, y+, y-, z+, z- (as ustar_at)
+                const T *row = X + (rl + 1) * N;
+                const C uc = (C)row[e];
+                const C lap = ((C)row[e + 1 < N ? e + 1 : 0] + (C)row[e > 0 ? e - 1 : N - 1] +
+                               (C)X[(rl + 2) * N + e] + (C)X[rl * N + e] + (C)v.zp + (C)v.zm - (C)6 * uc) /
+                              dxsq;
+                const C f = okf[(y / te1 - ty0) * FW + e / te0] ? (C)v.f : (C)0;
+                us = uc + (mu * lap * dt + f) / rho;
+            }
+            ((L *)&a[e * S + (rl >> 1)])[rl & 1] = (L)us;
         });
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
@@ -433,7 +493,8 @@ void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, 
     const size_t xc = n[0] / 2 + 1;
     const size_t xf = (size_t)(RB / 2 + 1) * n[0];
     const size_t xi = xf + xc * (RB + 1);
-    *lx = (size_t)twmax * twsz + std::max(xf, xi) * csz;
+    const size_t stage = (size_t)(RB + 2) * n[0] * (fp64 ? 8 : 4);  // Xf's rows y0-1 .. y0+RB
+    *lx = (size_t)twmax * twsz + std::max(xf * csz + stage, xi * csz);
     *ly = (size_t)twmax * twsz + (size_t)(RB + 1) * n[1] * csz;
     *lz = (size_t)twmax * twsz + (size_t)(3 * Q + 1) * n[2] * csz;
 }
