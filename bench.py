@@ -54,24 +54,27 @@ class HipEvents:
         return t.value
 
 
-def stage_bytes(w, s):
+def stage_bytes(w, s, passes=5):
     """Algorithmic HBM bytes per launch of each stage (s = bulk float size).
 
     Per-unit figures: a grid value is s bytes, a complex mode 2s; each pass
     reads its input once and writes its output once.  Beads: the 4x4x4
-    stencil of 3 components (192 values) plus FP64 state."""
+    stencil of 3 components (192 values) plus FP64 state.  With the plane
+    FFT path (passes == 3) the X and Y transforms of a plane are one pass, so
+    the Y stages move nothing of their own."""
     N = w.edge ** 3
     Nc = (w.edge // 2 + 1) * w.edge * w.edge
     M = w.M
     rec = 208 if s == 4 else 368  # spread record written per bead
+    ypass = 0 if passes == 3 else 2 * 3 * Nc * 2 * s
     return {
         "thermal_noise": 3 * Nc * 2 * s,
         "bead_update": M * (192 * s + 24 + 152 + rec),
         "spread": M * (24 + 3 * s) + 192 * M * s,
         "ustar_fft_x": 3 * N * s + 3 * Nc * 2 * s,
-        "fft_y": 2 * 3 * Nc * 2 * s,
+        "fft_y": ypass,
         "fft_z_projection": 3 * 3 * Nc * 2 * s,
-        "ifft_y": 2 * 3 * Nc * 2 * s,
+        "ifft_y": ypass,
         "ifft_x": 3 * Nc * 2 * s + 3 * N * s,
         "bond_forces": M * (48 + 24 * 2 + 24),
     }
@@ -276,12 +279,14 @@ def main():
               flush=True)
         return
     print("fp32-bulk: %.4f ms/step" % (ms32 / args.steps), file=sys.stderr, flush=True)
+    passes32 = s32.fft_passes()
     stages32 = s32.stage_times(args.stage_steps)
     s32.close()
     # FP64 path on the same workload
     s64 = w.build_system(selm.PREC_FP64, stochastic=True, seed=2026)
     steps64 = max(1, args.steps // 2)
     ms64, wall64 = time_steps(s64, ev, steps64, max(1, args.warmup // 2), max(1, args.runs // 2))
+    passes64 = s64.fft_passes()
     stages64 = s64.stage_times(args.stage_steps)
     s64.close()
     print("fp64: %.4f ms/step" % (ms64 / steps64), file=sys.stderr, flush=True)
@@ -295,7 +300,7 @@ def main():
     avg32 = {k: v / nst for k, v in stages32.items()}
     avg64 = {k: v / nst for k, v in stages64.items()}
     dom = max(avg32, key=avg32.get)
-    b32 = stage_bytes(w, 4)
+    b32 = stage_bytes(w, 4, passes32)
     achieved = b32[dom] / (avg32[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_FP32[dom],
@@ -329,6 +334,7 @@ def main():
         "grid_point_steps_per_s": round(N * args.steps / (ms32 * 1e-3), 1),
         "fp64": {"value": round(bead_steps64, 1), "ms_per_step": round(per64, 5), "steps": steps64},
         "fp32_over_fp64": round(bead_steps32 / bead_steps64, 3),
+        "fft_passes": {"fp32": passes32, "fp64": passes64},
         "wall_s": {"fp32": round(wall32, 4), "fp64": round(wall64, 4)},
         "stage_ms_fp32": {k: round(v, 5) for k, v in avg32.items()},
         "stage_ms_fp64": {k: round(v, 5) for k, v in avg64.items()},

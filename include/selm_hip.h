@@ -161,6 +161,12 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
  *  stages 1-4; here every stage runs alone. */
 int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
 
+/* Passes of the 3-D FFT in the ctx's step: 3 (plane path: XY, Z, XY; stages
+ * 4 and 6 are then empty), 5 (transposing X, Y, Z, Y, X) or 0 (runtime-length
+ * kernels); negative on error.  No reference counterpart: the reference's
+ * cuFFT plans (cuda/SELM_Eulerian_Period.cu:26-30) are opaque. */
+int selm_fft_passes(selm_ctx *ctx);
+
 /* Sum over the grid of u* for each component before the k=0 mode is
  * removed in the last step (FP64): rho * dx^3 * this equals the fluid
  * momentum injected by the beads' drag and thermal kicks. */

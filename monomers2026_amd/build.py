@@ -39,9 +39,18 @@ def _run(cmd, verbose):
 
 
 def build_lib(verbose=False):
+    """Each source is compiled to an object in parallel, then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o") for src in LIB_SOURCES]
+    cmds = [[HIPCC] + DEVICE_FLAGS + ["-c", "-I", INCLUDE, "-I", CSRC, "-o", o, src]
+            for src, o in zip(LIB_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + DEVICE_FLAGS + ["-shared", "-I", INCLUDE, "-I", CSRC, "-o", tmp] + LIB_SOURCES
-    _run(cmd, verbose)
+    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -15,12 +15,37 @@ struct cplx {
     T x, y;
 };
 
+// Device arrays reached through the replica descriptor are generic pointers
+// to the compiler, which then emits flat loads and stores: 64-bit address
+// arithmetic per access, and every LDS wait also waits for them.  Casting
+// such a pointer to the global address space gives global loads with a
+// scalar base and 32-bit offsets.
+#define SELM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ SELM_GLOBAL T *gbl(T *p) {
+    return (SELM_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ const SELM_GLOBAL T *gbl(const T *p) {
+    return (const SELM_GLOBAL T *)p;
+}
+
 template <typename T>
 __device__ __forceinline__ cplx<T> mk(T a, T b) {
     cplx<T> r;
     r.x = a;
     r.y = b;
     return r;
+}
+// complex load / store through a global-address-space pointer
+template <typename T>
+__device__ __forceinline__ cplx<T> ldc(const SELM_GLOBAL cplx<T> *p) {
+    return mk<T>(p->x, p->y);
+}
+template <typename T>
+__device__ __forceinline__ void stc(SELM_GLOBAL cplx<T> *p, cplx<T> v) {
+    p->x = v.x;
+    p->y = v.y;
 }
 template <typename T>
 __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
@@ -143,6 +168,36 @@ __device__ __forceinline__ double peskin_phi(double r, int f4) {
 __device__ __forceinline__ int wrap_idx(int p, int n) {
     int q = p % n;
     return q < 0 ? q + n : q;
+}
+
+// wrap_idx for the indices a stencil or a tile scan meets, p in [-n, 2n),
+// without an integer division; anything else takes the general modulo.
+__device__ __forceinline__ int wrap_near(int p, int n) {
+    int q = p + (p < 0 ? n : 0);
+    q -= q >= n ? n : 0;
+    if ((unsigned)q >= (unsigned)n) q = wrap_idx(p, n);
+    return q;
+}
+
+// floor(a / n) for 0 <= a < 2^22, n >= 1, given rn = 1.0f / n.  The FP32
+// quotient is within 1/2 of a/n, so its truncation is off by at most one,
+// which the remainder test corrects: no integer division.
+__device__ __forceinline__ int div_small(int a, int n, float rn) {
+    int q = (int)((float)a * rn);
+    const int r = a - q * n;
+    q += r >= n ? 1 : 0;
+    q -= r < 0 ? 1 : 0;
+    return q;
+}
+
+// floor(a / n) for 0 <= a < 2^31, n >= 1, given rn = 1.0 / n: as div_small
+// with an FP64 quotient.
+__device__ __forceinline__ int div_pos(int a, int n, double rn) {
+    int q = (int)((double)a * rn);
+    const int r = a - q * n;
+    q += r >= n ? 1 : 0;
+    q -= r < 0 ? 1 : 0;
+    return q;
 }
 
 // ---------------------------------------------------------------------------

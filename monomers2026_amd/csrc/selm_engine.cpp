@@ -608,6 +608,11 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
     return SELM_OK;
 }
 
+static bool env_is_zero(const char *name) {
+    const char *v = std::getenv(name);
+    return v && v[0] == '0' && v[1] == 0;
+}
+
 // ---------------------------------------------------------------------------
 // A batch of replicas stepped together
 // ---------------------------------------------------------------------------
@@ -615,6 +620,7 @@ struct Batch {
     bool fp64 = true;
     bool use_graph = true;
     bool force_generic_fft = false;
+    bool allow_plane = true;        // three-pass plane FFT path where the shapes allow it
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -671,6 +677,31 @@ struct Batch {
             fft_fast_lds(fp64, reps[r]->G.n, twmax, &a, &b, &c);
             fast = fast && fft_fast_supported(reps[r]->G.n) && a <= kLdsMax && b <= kLdsMax && c <= kLdsMax;
         }
+        // the three-pass plane path (FP32-bulk) when every replica has
+        // nx == ny <= 128 and a plane fits in the device's LDS per workgroup
+        bool plane = fast && allow_plane && !fp64;
+        int okmax = 0;
+        size_t lxy = 0, lzq = 0;
+        for (int r = 0; r < nrep && plane; r++) {
+            plane = fft_plane_shape(reps[r]->G.n);
+            okmax = std::max(okmax, fft_plane_okf(reps[r]->G.tn));
+        }
+        if (plane) {
+            for (int r = 0; r < nrep; r++) {
+                size_t a, b;
+                fft_plane_lds(fp64, reps[r]->G.n, twmax, okmax, &a, &b);
+                lxy = std::max(lxy, a);
+                lzq = std::max(lzq, b);
+            }
+            int dev = 0, lim = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            plane = lxy <= (size_t)lim && lzq <= kLdsMax;
+            if (plane && lxy > kLdsMax && fft_plane_prepare(fp64, twmax, lxy) != hipSuccess) {
+                (void)hipGetLastError();
+                plane = false;
+            }
+        }
         for (int r = 0; r < nrep; r++) {
             const Replica &R = *reps[r];
             const Geometry &G = R.G;
@@ -681,7 +712,15 @@ struct Batch {
             sen[r + 1] = sen[r] + G.N;
             smo[r + 1] = smo[r] + G.Nc;
             any_noise = any_noise || R.D.noise;
-            if (fast) {
+            if (plane) {
+                long long ixy, iz;
+                fft_plane_items(fp64, G.n, &ixy, &iz);
+                sfx[r + 1] = sfx[r] + ixy;
+                sfy[r + 1] = sfy[r];
+                sfz[r + 1] = sfz[r] + iz;
+                lfx = lxy;
+                lfz = lzq;
+            } else if (fast) {
                 long long ix, iy, iz;
                 fft_fast_items(fp64, G.n, &ix, &iy, &iz);
                 sfx[r + 1] = sfx[r] + ix;
@@ -702,7 +741,7 @@ struct Batch {
                 lfz = std::max(lfz, lds_z(G, twmax, csz));
             }
         }
-        if (lfx > kLdsMax || lfy > kLdsMax || lfz > kLdsMax) {
+        if ((!plane && lfx > kLdsMax) || lfy > kLdsMax || lfz > kLdsMax) {
             err = "batch exceeds the LDS budget";
             return SELM_ERR_UNSUPPORTED;
         }
@@ -737,6 +776,8 @@ struct Batch {
         L.lds_fz = lfz;
         L.twmax = twmax;
         L.fast = fast;
+        L.plane = plane;
+        L.okmax = okmax;
         dirty = false;
         return SELM_OK;
     }
@@ -878,6 +919,8 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     c->batch.fp64 = fp64;
     // SELM_FFT_GENERIC=1 selects the runtime-length FFT kernels for every size
     c->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
+    // SELM_FFT_PLANES=0 keeps the five-pass transposing path for every shape
+    c->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
     if (c->batch.init_streams(err) != SELM_OK) {
         c->batch.release();
         delete c;
@@ -1075,6 +1118,14 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes) {
     return SELM_OK;
 }
 
+int selm_fft_passes(selm_ctx *c) {
+    CTX_CHECK(c);
+    int rc = c->batch.ensure(c->err);
+    if (rc) return -rc;
+    if (!c->batch.L.fast) return 0;
+    return c->batch.L.plane ? 3 : 5;
+}
+
 int selm_stage_times(selm_ctx *c, int nsteps, double *ms, int nstages) {
     CTX_CHECK(c);
     std::string &err = c->err;
@@ -1192,6 +1243,7 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->batch.fp64 = pp->precision == SELM_PREC_FP64;
     P->batch.use_graph = pp->use_graph != 0;
     P->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
+    P->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
     if (P->batch.init_streams(P->err) != SELM_OK) {
         P->batch.release();
         delete P;

@@ -50,8 +50,8 @@ struct FastCfg<double> {
 };
 
 __device__ __forceinline__ void tw_to_lds(cplx<C> *dst, const void *src, int n) {
-    const cplx<C> *s = (const cplx<C> *)src;
-    for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = s[m];
+    const SELM_GLOBAL cplx<C> *s = gbl((const cplx<C> *)src);
+    for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = ldc(s + m);
 }
 
 template <typename T>
@@ -386,6 +386,9 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C>
     auto nost = [](int, int, cplx<C>) {};
     fft_run<C, L, N, S, P, NT, false, false>(a, tw, +1, nold, nost);
     T *out = (T *)R.u + ((long long)d * nz + z) * ny * N;
+    // the step is complete once this pass ends: advance the replica's counter
+    // (no kernel that runs from here to the end of the step reads it)
+    if (item == 0 && threadIdx.x == 0) R.ts[0] += 1;
     for (int idx = threadIdx.x; idx < RB * N; idx += NT) {
         const int rl = idx / N, e = idx - rl * N;
         const int y = y0 + rl;
@@ -395,12 +398,373 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C>
     }
 }
 
+// ------------------------------------------------------------------ plane path
+// When the x and y edges of every replica are equal (N <= 128) and a whole
+// xy-plane fits in LDS, the X and Y transforms of a plane run in one
+// workgroup and the step takes three passes instead of five:
+//
+//   pass   reads                      transform               writes
+//   XYf    u^n plane z, u*            r2c along x, DFT y      P[d][z][kx][ky]
+//   Zq     P (Q consecutive ky per    DFT along z, thermal    P in place
+//          (d, z)), thermal forcing   forcing, projection,
+//                                     k=0 removal, 1/N, iDFT
+//   XYi    P plane z                  iDFT y, c2r along x     u^{n+1} plane z
+//
+// P lives in spec; the thermal forcing nbuf is stored as [d][kz][kx][ky] to
+// match.  Both XY passes keep the plane in one LDS buffer of N x (N/2+1)
+// elements (element-major, stride N/2+1): the u plane is staged in it, u* is
+// formed into registers and written back as N/2 complex sequences (rows 2p
+// and 2p+1 as real and imaginary parts), transformed along x, unpacked into
+// the half spectrum of every row, and transformed along y.  The x-frequency
+// columns kx = 0 and kx = N/2 of a real plane are real sequences in y, so
+// they share one complex sequence (column 0 holds X(0) + i X(N/2)) and the Y
+// transform runs on N/2 sequences; the two are separated on output.
+template <int NMAX>
+struct PlaneNT {
+    static constexpr int v = NMAX <= 64 ? 256 : 512;
+};
+// The XY kernels are built for FP32-bulk only: two 512-thread workgroups per
+// CU (a 128-edge plane is 66.5 KB of LDS) need <= 128 VGPRs, i.e. 4 waves per
+// SIMD.  An FP64 plane takes 133 KB and its FP64 staging does not fit the
+// register budget; FP64 keeps the transposing passes.
+
+// Elements [0, COUNT) of ld, one thread's share (i = tid + k NT) held in
+// registers as V, then handed to st after the workgroup barrier: for
+// rewriting an LDS buffer in place under a new index map.
+template <int NT, int COUNT, class V, class LD, class ST>
+__device__ __forceinline__ void lds_remap(const LD &ld, const ST &st) {
+    constexpr int IT = (COUNT + NT - 1) / NT;
+    V buf[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < COUNT) buf[k] = ld(i);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < COUNT) st(i, buf[k]);
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int NT>
+__device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const cplx<C> *twx,
+                                         const cplx<C> *twy, int *okf, int2 *tab, cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
+    static_assert(sizeof(T) == 4 && sizeof(L) == 4, "the plane path is FP32-bulk");
+    constexpr int P = N / 2, S = P + 1, NN = N * N, IT = (NN + NT - 1) / NT;
+    const int nz = R.n[2];
+    const int z = (int)(item % nz), d = (int)(item / nz);
+    const int step = (int)(R.ts[0] + 1);
+    const SELM_GLOBAL T *ud = gbl((const T *)R.u) + d * R.N;
+    T *U = (T *)a;  // plane z of u_d, [y][x]
+
+    // stamps of the tile row under plane z, and the pool addressing split
+    // into a row part and a column part: the spread value of cell (x, y) is
+    // pool[row(y).x + col(x).x], valid when okf[row(y).y + col(x).y]
+    const int te0 = R.te[0], te1 = R.te[1], te2 = R.te[2], tn0 = R.tn[0], tn1 = R.tn[1];
+    const int tz = z / te2;
+    const int tcells = te0 * te1 * te2;
+    const SELM_GLOBAL int *stamp = gbl((const int *)R.tstamp) + tz * tn0 * tn1;
+    for (int i = threadIdx.x; i < tn0 * tn1; i += NT) okf[i] = stamp[i] == step;
+    int2 *rowt = tab, *colt = tab + N;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        const int ty = i / te1, tx = i / te0;
+        rowt[i] = make_int2(((tz * tn1 + ty) * tn0 * 3 + d) * tcells + ((z - tz * te2) * te1 + (i - ty * te1)) * te0,
+                            ty * tn0);
+        colt[i] = make_int2(tx * 3 * tcells + (i - tx * te0), tx);
+    }
+    const SELM_GLOBAL T *uz = ud + (long long)z * NN;
+    copy_batched<NT, NN, (IT < 16 ? IT : 16)>([&](int idx) { return uz[idx]; }, [&](int idx, T v) { U[idx] = v; });
+    __syncthreads();
+
+    // u* (the arithmetic of ustar_at, cuda/SELM_Kernel.cu:155-195) of this
+    // thread's elements, in registers; the z neighbours and the spread pool
+    // are loaded BT elements at a time.  u* = u + c_lap * (sum of the six
+    // neighbours - 6u) + c_f * f is evaluated in FP32: the Laplacian's terms
+    // are O(u) and enter scaled by mu dt / (rho dx^2) (0.01 at the headline
+    // workload), so the FP32 sums add well under one FP32 rounding of u*.
+    const SELM_GLOBAL T *up = ud + (long long)(z + 1 < nz ? z + 1 : 0) * NN;
+    const SELM_GLOBAL T *um = ud + (long long)(z > 0 ? z - 1 : nz - 1) * NN;
+    const SELM_GLOBAL T *pool = gbl((const T *)R.pool);
+    const float c_lap = (float)(R.mu * R.dt / (R.rho * R.dx * R.dx)), c_f = (float)(1.0 / R.rho);
+    constexpr int BT = IT < 4 ? IT : 4;
+    L us[IT];
+#pragma unroll
+    for (int k0 = 0; k0 < IT; k0 += BT) {
+        // opaque to the optimiser, so that the index arithmetic of a batch is
+        // done in that batch rather than hoisted (and spilled) for all of them
+        int base = (int)threadIdx.x + k0 * NT;
+        asm volatile("" : "+v"(base));
+        XfLoads<T> v[BT];
+        int ok[BT];
+#pragma unroll
+        for (int j = 0; j < BT; j++) {
+            const int idx = min(base + j * NT, NN - 1);
+            const int y = idx / N, x = idx - y * N;
+            const int2 rt = rowt[y], ct = colt[x];
+            v[j].zp = up[idx];
+            v[j].zm = um[idx];
+            v[j].f = pool[rt.x + ct.x];
+            ok[j] = rt.y + ct.y;
+        }
+#pragma unroll
+        for (int j = 0; j < BT; j++) {
+            const int idx = base + j * NT;
+            if (k0 + j < IT && idx < NN) {
+                const int y = idx / N, x = idx - y * N;
+                // neighbour order x+, x-, y+, y-, z+, z- (as ustar_at)
+                const T *row = U + y * N;
+                const float uc = row[x];
+                const float nb = (((((row[x + 1 < N ? x + 1 : 0] + row[x > 0 ? x - 1 : N - 1]) +
+                                     U[(y + 1 < N ? y + 1 : 0) * N + x]) +
+                                    U[(y > 0 ? y - 1 : N - 1) * N + x]) +
+                                   v[j].zp) +
+                                  v[j].zm) -
+                                 6.0f * uc;
+                const float f = okf[ok[j]] ? v[j].f : 0.0f;
+                us[k0 + j] = uc + c_lap * nb + c_f * f;
+            }
+        }
+    }
+    __syncthreads();  // every thread is done with U
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int idx = (int)threadIdx.x + k * NT;
+        if (idx < NN) {
+            const int y = idx / N, x = idx - y * N;
+            ((L *)&a[x * S + (y >> 1)])[y & 1] = us[k];
+        }
+    }
+    __syncthreads();
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, P, NT, false, false>(a, twx, -1, nold, nost);
+
+    // unpack: element y of sequence s (1 <= s < P) = X_y(s); sequence 0 =
+    // X_y(0) + i X_y(P), both real.  Z_p(k) = X_2p(k) + i X_2p+1(k).
+    lds_remap<NT, N * P, cplx<L>>(
+        [&](int i) {
+            const int s = i % P, y = i / P, p = y >> 1;
+            if (s == 0) {
+                const cplx<L> z0 = a[p], zn = a[P * S + p];
+                return (y & 1) ? mk<L>(z0.y, zn.y) : mk<L>(z0.x, zn.x);
+            }
+            // in FP32: a sum of two FP32 values halved is rounded once, as in FP64
+            const cplx<L> zk = a[s * S + p];
+            const cplx<L> zm = a[(N - s) * S + p];
+            return (y & 1) ? mk<L>((zk.y + zm.y) * (L)0.5, (zm.x - zk.x) * (L)0.5)
+                           : mk<L>((zk.x + zm.x) * (L)0.5, (zk.y - zm.y) * (L)0.5);
+        },
+        [&](int i, cplx<L> v) {
+            const int s = i % P, y = i / P;
+            a[y * S + s] = v;
+        });
+    fft_run<C, L, N, S, P, NT, false, false>(a, twy, -1, nold, nost);
+
+    // P[d][z][kx][ky]; the kx = 0 and kx = P columns are separated from
+    // sequence 0: V(ky) = Y0(ky) + i YP(ky) with Y0, YP transforms of real
+    // sequences, so Y0 = (V(ky) + conj V(-ky))/2, YP = (V(ky) - conj V(-ky))/(2i)
+    SELM_GLOBAL cplx<T> *out = gbl((cplx<T> *)R.spec) + ((long long)d * nz + z) * (P + 1) * N;
+    for (int idx = threadIdx.x; idx < (P + 1) * N; idx += NT) {
+        const int kx = idx / N, ky = idx - kx * N;
+        cplx<L> o;
+        if (kx == 0 || kx == P) {
+            const cplx<L> V = a[ky * S];
+            const cplx<L> Vm = a[((N - ky) % N) * S];
+            o = kx == 0 ? mk<L>((V.x + Vm.x) * (L)0.5, (V.y - Vm.y) * (L)0.5)
+                        : mk<L>((V.y + Vm.y) * (L)0.5, (Vm.x - V.x) * (L)0.5);
+        } else {
+            o = a[ky * S + kx];
+        }
+        stc(out + idx, o);
+    }
+}
+
+// Z pass on the plane layout.  Workgroup: QZ consecutive (kx, ky) columns in
+// the flat order c = kx * ny + ky, all three components; for each (d, z)
+// the QZ values are consecutive in memory.  With QZ = 17 a 128^3 grid has
+// 490 workgroups, within the 512 that two per CU hold at once (16 per
+// column block would make 520, and the last 8 would run after the others).
+// The arithmetic is that of z_body (thermal forcing, projection, k=0
+// removal, 1/N); the forcing of the thread's modes is loaded before the
+// forward transform so its latency is hidden behind it.
+constexpr int QZ = 17;
+
+template <typename T, int N>
+__device__ __forceinline__ void zq_body(const RepDev &R, long long item, const cplx<C> *tw,
+                                        cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
+    constexpr int Q = QZ, S = 3 * Q + 1, NT = FastCfg<T>::NTZ;
+    constexpr int IP = (N * Q + NT - 1) / NT;  // projection elements per thread
+    const int ny = R.n[1], xc = R.xc;
+    const int pl = xc * ny;  // columns of one (d, z) plane of P
+    const int c0 = (int)item * Q;
+    const int cv = min(Q, pl - c0);
+    SELM_GLOBAL cplx<T> *Ps = gbl((cplx<T> *)R.spec) + c0;
+    // sequence s = d*Q + q <-> column c0+q of component d; element e = z
+    copy_batched<NT, 3 * Q * N, 16>(
+        [&](int idx) {
+            const int q = idx % Q, dz = idx / Q;
+            const cplx<T> v = ldc(Ps + (long long)dz * pl + min(q, cv - 1));
+            return q < cv ? v : czero<T>();
+        },
+        [&](int idx, cplx<T> v) {
+            const int q = idx % Q, dz = idx / Q;
+            const int d = dz / N, e = dz - d * N;
+            a[e * S + d * Q + q] = ccast<L>(v);
+        });
+    cplx<T> nv[IP][3];
+    if (R.noise) {
+        const SELM_GLOBAL cplx<T> *nb = gbl((const cplx<T> *)R.nbuf) + c0;
+#pragma unroll
+        for (int k = 0; k < IP; k++) {
+            const int idx = min((int)threadIdx.x + k * NT, N * Q - 1);
+            const int q = min(idx % Q, cv - 1), e = idx / Q;
+#pragma unroll
+            for (int d = 0; d < 3; d++) nv[k][d] = ldc(nb + d * R.Nc + (long long)e * pl + q);
+        }
+    }
+    __syncthreads();
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, 3 * Q, NT, false, false>(a, tw, -1, nold, nost);
+
+    const C *kxt = (const C *)R.ktabd[0], *kyt = (const C *)R.ktabd[1], *kzt = (const C *)R.ktabd[2];
+    const C invN = 1.0 / (double)R.N;
+    const float rny = 1.0f / (float)ny;
+#pragma unroll
+    for (int kk = 0; kk < IP; kk++) {
+        const int idx = (int)threadIdx.x + kk * NT;
+        if (idx >= N * Q) continue;
+        const int q = idx % Q, e = idx / Q;
+        if (q >= cv) continue;
+        const int c = c0 + q;
+        const int i = div_small(c, ny, rny), j = c - i * ny, k = e;
+        cplx<C> w[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) w[d] = ccast<C>(a[e * S + d * Q + q]);
+        const C lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
+        if (R.noise) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) w[d] = cadd(w[d], ccast<C>(nv[kk][d]));
+        }
+        if (i == 0 && j == 0 && k == 0) {
+            if (R.diag) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) R.diag[d] = (double)w[d].x;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = ccast<L>(czero<C>());
+            continue;
+        }
+        const cplx<C> D0 = mk<C>(kxt[4 * i + 1], kxt[4 * i + 2]);
+        const cplx<C> D1 = mk<C>(kyt[4 * j + 1], kyt[4 * j + 2]);
+        const cplx<C> D2 = mk<C>(kzt[4 * k + 1], kzt[4 * k + 2]);
+        const cplx<C> sdiv = cadd(cadd(cmul(D0, w[0]), cmul(D1, w[1])), cmul(D2, w[2]));
+        const cplx<C> qv = mk<C>(sdiv.x / lam, sdiv.y / lam);
+        const cplx<C> G[3] = {mk<C>(-D0.x, D0.y), mk<C>(-D1.x, D1.y), mk<C>(-D2.x, D2.y)};
+#pragma unroll
+        for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = ccast<L>(cscale(csub(w[d], cmul(G[d], qv)), invN));
+    }
+    __syncthreads();
+    fft_run<C, L, N, S, 3 * Q, NT, false, false>(a, tw, +1, nold, nost);
+    for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
+        const int q = idx % Q, dz = idx / Q;
+        const int d = dz / N, e = dz - d * N;
+        if (q < cv) stc(Ps + (long long)dz * pl + q, ccast<T>(a[e * S + d * Q + q]));
+    }
+}
+
+template <typename T, int N, int NT>
+__device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const cplx<C> *twx,
+                                         const cplx<C> *twy, cplx<typename FastCfg<T>::L> *a) {
+    using L = typename FastCfg<T>::L;
+    constexpr int P = N / 2, S = P + 1, NN = N * N;
+    const int nz = R.n[2];
+    const int z = (int)(item % nz), d = (int)(item / nz);
+    const SELM_GLOBAL cplx<T> *in = gbl((const cplx<T> *)R.spec) + ((long long)d * nz + z) * (P + 1) * N;  // [kx][ky]
+    constexpr int NI = (P - 1) * N, ITI = (NI + NT - 1) / NT, I0 = (N + NT - 1) / NT;
+    // sequence 0 = Y0h + i YPh, the Hermitian parts of the kx = 0 and kx = P
+    // columns (what a c2r keeps of them), whose inverses are real; its loads
+    // are issued first and complete under the batched copy below
+    cplx<T> c0[I0][4];
+#pragma unroll
+    for (int k = 0; k < I0; k++) {
+        const int ky = min((int)threadIdx.x + k * NT, N - 1), km = (N - ky) % N;
+        c0[k][0] = ldc(in + ky);
+        c0[k][1] = ldc(in + km);
+        c0[k][2] = ldc(in + P * N + ky);
+        c0[k][3] = ldc(in + P * N + km);
+    }
+    // columns 1 <= kx < P: sequence kx of the inverse Y transform
+    copy_batched<NT, NI, (ITI < 16 ? ITI : 16)>([&](int idx) { return ldc(in + N + idx); },
+                                              [&](int idx, cplx<T> v) {
+                                                  const int kx = 1 + idx / N, ky = idx % N;
+                                                  a[ky * S + kx] = ccast<L>(v);
+                                              });
+#pragma unroll
+    for (int k = 0; k < I0; k++) {
+        const int ky = (int)threadIdx.x + k * NT;
+        if (ky < N) {
+            const cplx<T> A = c0[k][0], Am = c0[k][1], B = c0[k][2], Bm = c0[k][3];
+            const cplx<L> A0 = mk<L>((A.x + Am.x) * (L)0.5, (A.y - Am.y) * (L)0.5);
+            const cplx<L> B0 = mk<L>((B.x + Bm.x) * (L)0.5, (B.y - Bm.y) * (L)0.5);
+            a[ky * S] = mk<L>(A0.x - B0.y, A0.y + B0.x);
+        }
+    }
+    __syncthreads();
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, P, NT, false, false>(a, twy, +1, nold, nost);
+
+    // pack rows 2p, 2p+1 as Z_e = A_e + i B_e with the Hermitian extension
+    // along x; element y of sequence 0 is X_y(0) + i X_y(P)
+    lds_remap<NT, N * P, cplx<L>>(
+        [&](int i) {
+            const int e = i % N, p = i / N;  // consecutive lanes read consecutive elements
+            const int y0 = 2 * p, y1 = 2 * p + 1;
+            cplx<L> A, B;
+            if (e == 0 || e == P) {
+                const cplx<L> W0 = a[y0 * S], W1 = a[y1 * S];
+                A = mk<L>(e == 0 ? W0.x : W0.y, (L)0);
+                B = mk<L>(e == 0 ? W1.x : W1.y, (L)0);
+            } else if (e < P) {
+                A = a[y0 * S + e];
+                B = a[y1 * S + e];
+            } else {
+                A = cconj(a[y0 * S + N - e]);
+                B = cconj(a[y1 * S + N - e]);
+            }
+            return mk<L>(A.x - B.y, A.y + B.x);
+        },
+        [&](int i, cplx<L> v) {
+            const int e = i % N, p = i / N;
+            a[e * S + p] = v;
+        });
+    fft_run<C, L, N, S, P, NT, false, false>(a, twx, +1, nold, nost);
+    SELM_GLOBAL T *out = gbl((T *)R.u) + ((long long)d * nz + z) * NN;
+    // the step is complete once this pass ends: advance the replica's counter
+    // (no kernel that runs from here to the end of the step reads it)
+    if (item == 0 && threadIdx.x == 0) R.ts[0] += 1;
+    for (int idx = threadIdx.x; idx < NN; idx += NT) {
+        const int y = idx / N, x = idx - y * N;
+        const cplx<L> v = a[x * S + (y >> 1)];
+        out[idx] = (T)((y & 1) ? v.y : v.x);
+    }
+}
+
 // ------------------------------------------------------------------ thermal forcing
 // Thread per mode, in the row order of C ([kx][ky][kz], kz fastest).  Runs on
 // a forked stream while the beads are updated and spread.
+// With plane = 1 the order is that of the plane path's P ([kz][kx][ky], ky
+// fastest).
 template <typename T>
 __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, int nrep,
-                                               const long long *__restrict__ start, long long total) {
+                                               const long long *__restrict__ start, long long total,
+                                               int plane) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
@@ -408,25 +772,35 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
     if (!R.noise) return;
     const long long m = g - start[r];
     const int ny = R.n[1], nz = R.n[2];
-    const int k = (int)(m % nz);
-    const long long ij = m / nz;
-    const int j = (int)(ij % ny), i = (int)(ij / ny);
+    const int mi = (int)m;  // Nc < 2^31
+    int i, j, k;
+    if (plane) {
+        const int ki = div_pos(mi, ny, 1.0 / ny);
+        j = mi - ki * ny;
+        k = div_pos(ki, R.xc, 1.0 / R.xc);
+        i = ki - k * R.xc;
+    } else {
+        const int ij = div_pos(mi, nz, 1.0 / nz);
+        k = mi - ij * nz;
+        i = div_pos(ij, ny, 1.0 / ny);
+        j = ij - i * ny;
+    }
     const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
     const T lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
     cplx<T> gv[3];
     thermal_mode<T>(R, i, j, k, lam, (uint32_t)(R.ts[0] + 1), gv);
-    cplx<T> *nb = (cplx<T> *)R.nbuf;
+    SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
 #pragma unroll
-    for (int d = 0; d < 3; d++) nb[d * R.Nc + m] = gv[d];
+    for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
 }
 
 // ------------------------------------------------------------------ kernels
 // Each kernel is instantiated per size class NMAX (64, 128, 256) and holds
 // only the lengths up to NMAX.  The register budget is then set by the
 // largest length the batch can hold, not by 256.
-#define SELM_FFT_KERNEL(kname, axis, BODY, NTH)                                                       \
+#define SELM_FFT_KERNEL(kname, axis, BODY, NTH, ...)                                                  \
     template <typename T, int NMAX>                                                                   \
-    __global__ __launch_bounds__(FastCfg<T>::NTH) void kname(const RepDev *__restrict__ reps, int nrep, \
+    __global__ __launch_bounds__(FastCfg<T>::NTH) __VA_ARGS__ void kname(const RepDev *__restrict__ reps, int nrep, \
                                                             const long long *__restrict__ start,      \
                                                             long long total, int twmax) {             \
         extern __shared__ __align__(16) unsigned char smem[];                                         \
@@ -455,7 +829,46 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
 #define SELM_CASE_XI(n) \
     case n: if constexpr (n <= NMAX) xi_body<T, n>(R, item, tw, a); break;
 
+#define SELM_CASE_ZQ(n) \
+    case n: if constexpr (n <= NMAX) zq_body<T, n>(R, item, tw, a); break;
+
 SELM_FFT_KERNEL(k_xf, 0, SELM_CASE_XF, NT)
+// 4 waves per SIMD: two 512-thread workgroups per CU
+SELM_FFT_KERNEL(k_zq, 2, SELM_CASE_ZQ, NTZ, __attribute__((amdgpu_waves_per_eu(4))))
+
+// XY passes of the plane path.  Dynamic LDS: the x and y twiddles (twmax
+// slots each), the tile stamps of one plane (okmax ints), then the plane.
+#define SELM_PLANE_KERNEL(kname, BODY, ...)                                                            \
+    template <typename T, int NMAX>                                                                   \
+    __global__ __launch_bounds__(PlaneNT<NMAX>::v) __attribute__((amdgpu_waves_per_eu(4)))           \
+    void kname(const RepDev *__restrict__ reps, int nrep,                                             \
+                                                             const long long *__restrict__ start,     \
+                                                             long long total, int twmax, int okmax) { \
+        extern __shared__ __align__(16) unsigned char smem[];                                         \
+        const long long g = xcd_remap(blockIdx.x, gridDim.x);                                         \
+        if (g >= total) return;                                                                       \
+        const int r = find_rep(start, nrep, g);                                                       \
+        const RepDev &R = reps[r];                                                                    \
+        const long long item = g - start[r];                                                          \
+        cplx<C> *twx = (cplx<C> *)smem, *twy = twx + twmax;                                           \
+        int2 *tab = (int2 *)(twy + twmax);                                                            \
+        int *okf = (int *)(tab + 2 * twmax);                                                          \
+        auto *a = (cplx<typename FastCfg<T>::L> *)(okf + okmax);                                      \
+        tw_to_lds(twx, R.twd[0], R.n[0]);                                                             \
+        tw_to_lds(twy, R.twd[1], R.n[1]);                                                             \
+        (void)okf;                                                                                    \
+        (void)tab;                                                                                    \
+        switch (R.n[0]) {                                                                             \
+            SELM_FFT_FOR_EACH_N(BODY)                                                                 \
+            default: break;                                                                           \
+        }                                                                                             \
+    }
+#define SELM_CASE_XYF(n) \
+    case n: if constexpr (n <= NMAX) xyf_body<T, n, PlaneNT<NMAX>::v>(R, item, twx, twy, okf, tab, a); break;
+#define SELM_CASE_XYI(n) \
+    case n: if constexpr (n <= NMAX) xyi_body<T, n, PlaneNT<NMAX>::v>(R, item, twx, twy, a); break;
+SELM_PLANE_KERNEL(k_xyf, SELM_CASE_XYF)
+SELM_PLANE_KERNEL(k_xyi, SELM_CASE_XYI)
 SELM_FFT_KERNEL(k_yf, 1, SELM_CASE_YF, NT)
 SELM_FFT_KERNEL(k_zp, 2, SELM_CASE_Z, NTZ)
 SELM_FFT_KERNEL(k_yi, 1, SELM_CASE_YI, NT)
@@ -501,6 +914,81 @@ void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, 
 
 static inline unsigned pad8(long long n) { return (unsigned)((n + 7) & ~7LL); }
 
+// ---- plane path
+// A replica can take the plane path when nx == ny <= 128 (both compile-time
+// lengths) and nz has a compile-time length; the XY passes then need the LDS
+// below, which the caller checks against the device's per-workgroup limit.
+bool fft_plane_shape(const int n[3]) {
+    return n[0] == n[1] && n[0] <= 128 && fast_len(n[0]) && fast_len(n[2]);
+}
+
+int fft_plane_okf(const int tn[3]) { return (tn[0] * tn[1] + 3) & ~3; }
+
+// LDS bytes of the XY passes and of the Z pass of the plane path
+void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, size_t *lxy, size_t *lz) {
+    const size_t csz = fp64 ? sizeof(cplx<FastCfg<double>::L>) : sizeof(cplx<FastCfg<float>::L>);
+    const size_t tsz = fp64 ? 8 : 4;
+    const size_t N = (size_t)n[0], P = N / 2;
+    const int Q = QZ;
+    // twiddles x and y, the row and column pool tables, the tile stamps, the plane
+    *lxy = 2 * (size_t)twmax * sizeof(cplx<C>) + 2 * (size_t)twmax * 8 + (size_t)okmax * 4 +
+           std::max(N * (P + 1) * csz, N * N * tsz);
+    *lz = (size_t)twmax * sizeof(cplx<C>) + (size_t)(3 * Q + 1) * n[2] * csz;
+}
+
+// workgroups of the XY passes and of the Z pass for one replica
+void fft_plane_items(bool fp64, const int n[3], long long *ixy, long long *iz) {
+    (void)fp64;
+    *ixy = 3LL * n[2];
+    *iz = ((long long)(n[0] / 2 + 1) * n[1] + QZ - 1) / QZ;
+}
+
+// Allow the XY kernels of the size class their dynamic LDS (above 64 KiB).
+template <typename T>
+static hipError_t plane_attr_t(int nmax, size_t lds) {
+    hipError_t e;
+    if (nmax <= 64) {
+        e = hipFuncSetAttribute((const void *)k_xyf<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_xyi<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    } else {
+        e = hipFuncSetAttribute((const void *)k_xyf<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_xyi<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
+    return e;
+}
+
+hipError_t fft_plane_prepare(bool fp64, int twmax, size_t lds) {
+    return fp64 ? hipErrorNotSupported : plane_attr_t<float>(twmax, lds);
+}
+
+template <typename T, int NMAX>
+static void launch_plane_n(const StepLaunch &L, int stage, hipStream_t st) {
+    constexpr int NT = PlaneNT<NMAX>::v;
+    switch (stage) {
+        case kStageFftX:
+            hipLaunchKernelGGL((k_xyf<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep,
+                               L.st_fx, L.tot_fx, L.twmax, L.okmax);
+            break;
+        case kStageFftZ:
+            hipLaunchKernelGGL((k_zq<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st, L.reps,
+                               L.nrep, L.st_fz, L.tot_fz, L.twmax);
+            break;
+        case kStageFftXi:
+            hipLaunchKernelGGL((k_xyi<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep,
+                               L.st_fx, L.tot_fx, L.twmax, L.okmax);
+            break;
+        default: break;  // the Y passes are folded into the XY passes
+    }
+}
+
+template <typename T>
+static void launch_plane_t(const StepLaunch &L, int stage, hipStream_t st) {
+    if (L.twmax <= 64) launch_plane_n<T, 64>(L, stage, st);
+    else launch_plane_n<T, 128>(L, stage, st);
+}
+
 template <typename T, int NMAX>
 static void launch_fast_n(const StepLaunch &L, int stage, hipStream_t st) {
     const int NT = FastCfg<T>::NT;
@@ -540,11 +1028,18 @@ static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
 void launch_noise(const StepLaunch &L, hipStream_t st) {
     if (L.tot_modes <= 0) return;
     const unsigned nb = (unsigned)((L.tot_modes + 255) / 256);
-    if (L.fp64) hipLaunchKernelGGL(k_noise<double>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes);
-    else hipLaunchKernelGGL(k_noise<float>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes);
+    const int plane = L.plane ? 1 : 0;
+    if (L.fp64)
+        hipLaunchKernelGGL(k_noise<double>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes, plane);
+    else
+        hipLaunchKernelGGL(k_noise<float>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes, plane);
 }
 
 void launch_fft_fast(const StepLaunch &L, int stage, hipStream_t st) {
+    if (L.plane) {  // FP32-bulk only
+        launch_plane_t<float>(L, stage, st);
+        return;
+    }
     if (L.fp64) launch_fast_t<double>(L, stage, st);
     else launch_fast_t<float>(L, stage, st);
 }

@@ -64,7 +64,7 @@ template <typename T>
 __device__ __forceinline__ void thermal_mode(const RepDev &R, int i, int j, int k, T lam, uint32_t step,
                                              cplx<T> g[3]) {
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2], xc = R.xc;
-    const int k1 = (nz - k) % nz, j1 = (ny - j) % ny, i1 = (nx - i) % nx;
+    const int k1 = k == 0 ? 0 : nz - k, j1 = j == 0 ? 0 : ny - j, i1 = i == 0 ? 0 : nx - i;  // -k mod n
     const bool selfconj = (k1 == k && j1 == j && i1 == i);
     const bool plane = (i == 0) || (2 * i == nx);
     const T coef = sqrt(-lam * (T)R.inv_dxsq);

@@ -28,9 +28,10 @@ namespace selm {
 __device__ __forceinline__ int axis_tiles(double s, int n, int te, int *out) {
     const int lo = (int)(floor(s - 0.5) - 1.0);
     const int hi = (int)floor(s) + 2;
+    const float rte = 1.0f / (float)te;
     int cnt = 0;
     for (int c = lo; c <= hi && c - lo < 6; c++) {
-        const int t = wrap_idx(c, n) / te;
+        const int t = div_small(wrap_near(c, n), te, rte);
         bool dup = false;
         for (int q = 0; q < cnt; q++) dup |= out[q] == t;
         if (!dup) out[cnt++] = t;
@@ -73,14 +74,15 @@ __global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__re
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const long long N = R.N;
     const int f4 = R.f4;
-    SpreadRec<T> *rec = (SpreadRec<T> *)R.brec + b;
+    SELM_GLOBAL SpreadRec<T> *rec = gbl((SpreadRec<T> *)R.brec) + b;
+    const SELM_GLOBAL double *gx = gbl((const double *)R.x);
 
     double s[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) s[d] = (R.x[3 * b + d] - R.lo) / R.dx;
+    for (int d = 0; d < 3; d++) s[d] = (gx[3 * b + d] - R.lo) / R.dx;
 
     const int li = sl & 3, lj = sl >> 2;
-    const T *u = (const T *)R.u;
+    const SELM_GLOBAL T *u = gbl((const T *)R.u);
     double gam[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -91,13 +93,13 @@ __global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__re
         const int p0 = (int)(floor(rx) - 1.0), p1 = (int)(floor(ry) - 1.0), p2 = (int)(floor(rz) - 1.0);
         const T fx = phi_t<T>(fabs(rx - (p0 + li)), f4);
         const T fy = phi_t<T>(fabs(ry - (p1 + lj)), f4);
-        const long long col = (long long)wrap_idx(p1 + lj, ny) * nx + wrap_idx(p0 + li, nx);
-        const T *ud = u + d * N;
+        const long long col = (long long)wrap_near(p1 + lj, ny) * nx + wrap_near(p0 + li, nx);
+        const SELM_GLOBAL T *ud = u + d * N;
         T fz[4], uv[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             fz[k] = phi_t<T>(fabs(rz - (p2 + k)), f4);
-            uv[k] = ud[(long long)wrap_idx(p2 + k, nz) * ny * nx + col];
+            uv[k] = ud[(long long)wrap_near(p2 + k, nz) * ny * nx + col];
         }
         T acc = (T)0;
 #pragma unroll
@@ -110,9 +112,9 @@ __global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__re
             if (li == 0) rec->w[d][1][lj] = fy;
             if (sl < 4) rec->w[d][2][sl] = sl == 0 ? fz[0] : (sl == 1 ? fz[1] : (sl == 2 ? fz[2] : fz[3]));
             if (sl == 0) {
-                rec->p[d][0] = wrap_idx(p0, nx);
-                rec->p[d][1] = wrap_idx(p1, ny);
-                rec->p[d][2] = wrap_idx(p2, nz);
+                rec->p[d][0] = wrap_near(p0, nx);
+                rec->p[d][1] = wrap_near(p1, ny);
+                rec->p[d][2] = wrap_near(p2, nz);
             }
         }
     }
@@ -127,16 +129,18 @@ __global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__re
         const int cy = axis_tiles(s[1], ny, R.te[1], ty);
         const int cz = axis_tiles(s[2], nz, R.te[2], tz);
         const int ncomb = cx * cy * cz;
+        const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
         for (int q = sl; q < ncomb; q += 16) {
-            const int a = q % cx, qb = (q / cx) % cy, c = q / (cx * cy);
+            const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
+            const int a = q - qa * cx, qb = qa - c * cy;
             const int t = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
             const int slot = atomicAdd(&R.bcnt[t], 1);
             if (slot < kBucketCap) {
-                R.bucket[t * kBucketCap + slot] = b;
+                gbl(R.bucket)[t * kBucketCap + slot] = b;
             } else {
                 const int o = atomicAdd(&R.ctr[step & 1], 1);
-                R.ovf[2 * o] = t;
-                R.ovf[2 * o + 1] = b;
+                gbl(R.ovf)[2 * o] = t;
+                gbl(R.ovf)[2 * o + 1] = b;
             }
         }
     }
@@ -146,24 +150,24 @@ __global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__re
         const double sd = d == 0 ? s[0] : (d == 1 ? s[1] : s[2]);
         const int i = 3 * b + d;
         const double dt = R.dt;
-        const double vi = R.v[i];
+        const double vi = gbl(R.v)[i];
         double tf = -R.ups * (vi - gd) * dt;
         if (R.noise) {
             double z0, z1;
             normal_pair_f64(R.seed, kStreamBead, step, (uint32_t)i, 0u, z0, z1);
             tf += z0 * R.bead_coef;
         }
-        R.tf[i] = tf;
-        R.sg[i] = sd;
+        gbl(R.tf)[i] = tf;
+        gbl(R.sg)[i] = sd;
         rec->f[d] = (T)(-tf);
-        const double ft = tf + dt * R.fc[i];
-        R.ftot[i] = ft / dt;
+        const double ft = tf + dt * gbl(R.fc)[i];
+        gbl(R.ftot)[i] = ft / dt;
         double tmp = vi * dt;
-        tmp += R.x[i];
+        tmp += gx[i];
         if (tmp > R.hi) tmp -= R.L;
         else if (tmp < R.lo) tmp += R.L;
-        R.x[i] = tmp;
-        R.v[i] = vi + ft / R.m[b];
+        gbl(R.x)[i] = tmp;
+        gbl(R.v)[i] = vi + ft / gbl(R.m)[b];
     }
 }
 
@@ -200,7 +204,7 @@ __device__ __forceinline__ void spread_stage(SpreadRec<T> *dst, const SpreadRec<
     for (int q = 0; q < PER; q++) {
         const int idx = min(lt + q * NT, tot - 1);
         const int k = idx / V;
-        v[q] = ((const u32x4 *)&brec[sorted[base + k]])[idx - k * V];
+        v[q] = gbl((const u32x4 *)&brec[sorted[base + k]])[idx - k * V];
     }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -330,7 +334,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
 
     const int nb = min(cnt, kBucketCap);
     if (tid == 0) nlist = nb;
-    for (int i = tid; i < nb; i += kSpreadThreads) list[i] = R.bucket[t * kBucketCap + i];
+    for (int i = tid; i < nb; i += kSpreadThreads) list[i] = gbl(R.bucket)[t * kBucketCap + i];
     __syncthreads();
     if (cnt > kBucketCap) {
         const int novf = R.ctr[par];
@@ -377,7 +381,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
         __syncthreads();  // chunk c consumed, chunk c+1 staged
     }
     const int tcells = te0 * te1 * R.te[2];
-    T *pool = (T *)R.pool + (long long)t * 3 * tcells;
+    SELM_GLOBAL T *pool = gbl((T *)R.pool) + (long long)t * 3 * tcells;
     for (int c = tid; c < 3 * tcells; c += kSpreadThreads) {
         const int d = c / tcells, cell = d * kTileCells + (c - d * tcells);
         pool[c] = (T)acc[cell];
@@ -807,8 +811,9 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
             else launch_fft_generic<T>(L, stage, st);
             break;
         case kStageBonds:
+            // on the compile-time-length path the Xi pass advances the step counter
             hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
-                               reps, nrep, L.st_bead, L.tot_bead, 1);
+                               reps, nrep, L.st_bead, L.tot_bead, L.fast ? 0 : 1);
             break;
         default: break;
     }
@@ -845,18 +850,25 @@ void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
     else launch_stage_t<float>(L, stage, st);
 }
 
-// One step.  The thermal forcing (compile-time-length path) runs on a forked
-// stream concurrently with the bead update, spread and the first two FFT
-// passes, and joins before the Z pass that consumes it.
+// One step.  On the compile-time-length path the thermal forcing and the
+// conservative forces at x^{n+1} run on a forked stream once the bead update
+// is done, concurrently with the spread and the first two FFT passes; the
+// branch joins before the Z pass, which consumes the forcing.  The bead
+// update runs alone: it is latency-bound and slows down beside the forcing.
+// There the step counter is advanced by the last pass (Xi), after every
+// kernel that reads it; otherwise by the bond kernel at the end of the step.
 void launch_step(const StepLaunch &L, hipStream_t st) {
     const bool fork = L.tot_modes > 0;
+    launch_stage(L, kStageBead, st);
     if (fork) {
         (void)hipEventRecord(L.ev_fork, st);
         (void)hipStreamWaitEvent(L.st2, L.ev_fork, 0);
         launch_noise(L, L.st2);
+        launch_stage(L, kStageBonds, L.st2);
         (void)hipEventRecord(L.ev_join, L.st2);
     }
-    for (int s = kStageBead; s < kNumStages; s++) {
+    for (int s = kStageSpread; s < kNumStages; s++) {
+        if (fork && s == kStageBonds) continue;
         if (s == kStageFftZ && fork) (void)hipStreamWaitEvent(st, L.ev_join, 0);
         launch_stage(L, s, st);
     }

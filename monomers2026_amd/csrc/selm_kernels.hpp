@@ -31,6 +31,8 @@ struct StepLaunch {
     size_t lds_fx, lds_fy, lds_fz;   // dynamic LDS bytes per pass
     int twmax;                       // LDS twiddle slots (elements) preceding the data
     bool fast;                       // compile-time-length FFT kernels (selm_fft_kernels.hip)
+    bool plane;                      // ... as the three-pass plane path (XYf, Zq, XYi)
+    int okmax;                       // plane path: LDS ints for one plane's tile stamps
     const long long *st_modes;       // thermal-forcing work table (half-spectrum modes)
     long long tot_modes;             // 0 when no replica has thermal forcing (or not fast)
     hipStream_t st2;                 // forked stream for the thermal forcing
@@ -50,5 +52,10 @@ void fft_fast_items(bool fp64, const int n[3], long long *ix, long long *iy, lon
 void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, size_t *lz);
 void launch_fft_fast(const StepLaunch &L, int stage, hipStream_t st);
 void launch_noise(const StepLaunch &L, hipStream_t st);
+bool fft_plane_shape(const int n[3]);
+int fft_plane_okf(const int tn[3]);
+void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, size_t *lxy, size_t *lz);
+void fft_plane_items(bool fp64, const int n[3], long long *ixy, long long *iz);
+hipError_t fft_plane_prepare(bool fp64, int twmax, size_t lds);
 
 }  // namespace selm

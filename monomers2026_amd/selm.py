@@ -37,7 +37,7 @@ EXPORTS = [
     "selm_pool_create", "selm_pool_destroy", "selm_pool_last_error", "selm_pool_admit",
     "selm_pool_step", "selm_pool_sync", "selm_pool_get_beads", "selm_pool_get_field",
     "selm_pool_timestep", "selm_pool_retire", "selm_pool_stats", "selm_pool_stream",
-    "selm_msd", "selm_set_angles", "selm_set_pair_lj",
+    "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes",
 ]
 
 
@@ -99,6 +99,7 @@ def load(path=None):
     L.selm_footprint.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(C.c_size_t)]
     L.selm_stage_times.argtypes = [vp, C.c_int, dp, C.c_int]
     L.selm_mean_mode.argtypes = [vp, dp]
+    L.selm_fft_passes.argtypes = [vp]
     L.selm_stream.argtypes = [vp]
     L.selm_stream.restype = vp
     L.selm_pool_create.argtypes = [C.POINTER(PoolParams), C.POINTER(vp)]
@@ -270,6 +271,12 @@ class System:
         out = np.zeros(3)
         self._chk(self.L.selm_mean_mode(self.h, _d(out)), "mean_mode")
         return out
+
+    def fft_passes(self):
+        """3 (plane path), 5 (transposing path) or 0 (runtime-length kernels)."""
+        n = self.L.selm_fft_passes(self.h)
+        self._chk(-n if n < 0 else 0, "fft_passes")
+        return n
 
     @property
     def stream(self):

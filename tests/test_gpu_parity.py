@@ -185,3 +185,57 @@ def test_deterministic_repeat(gpu_lib):
         s.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128), (32, 32, 48), (24, 24, 20), (30, 30, 30)])
+def test_plane_vs_transposing_fft_fp32(gpu_lib, monkeypatch, dims):
+    """FP32-bulk, from a random (not divergence-free) u^n with beads and noise
+    on: the three-pass plane FFT (XY, Z, XY; nx == ny) and the five-pass
+    transposing FFT give the same steps to FP32 rounding (rel 2e-6 in u and
+    v after 3 steps), and the plane path's first step equals the FP64
+    oracle's to rel 2e-6.  (30, 30, 30) has an odd number (15) of row pairs
+    per plane; (32, 32, 48) and (24, 24, 20) have nz != nx."""
+    import oracle_lib as O
+    from monomers2026_amd import selm
+    rng = np.random.default_rng(7)
+    lo, hi = (0.0, 0.0, 0.0), tuple(float(d) for d in dims)
+    M = 60
+    x = rng.uniform(0, min(dims), (M, 3))
+    v = rng.standard_normal((M, 3))
+    m = rng.uniform(0.5, 2.0, M)
+    ij = np.array([(i, i + 1) for i in range(M - 1)], np.int32)
+    K = np.full(M - 1, 50.0)
+    r0 = np.full(M - 1, 1.5)
+    u0 = rng.standard_normal((3, dims[2], dims[1], dims[0]))
+    kw = dict(rho=1.3, mu=0.7, kB=1.0, T=2.0, dt=0.02, seed=41)
+    out = {}
+    for planes in ("1", "0"):
+        monkeypatch.setenv("SELM_FFT_PLANES", planes)
+        p = selm.make_params(dims, lo, hi, stochastic=True, precision=selm.PREC_FP32_BULK, **kw)
+        s = selm.System(p)
+        s.set_beads(x, v, m)
+        s.set_bonds(ij, K, r0)
+        s.init()
+        s.set_field(u0)
+        assert s.fft_passes() == (3 if planes == "1" else 5)
+        s.step(1)
+        if planes == "1":
+            o = O.Oracle(*dims, lo, hi, noise=True, **kw)
+            o.set_beads(x, v, m)
+            o.set_bonds(ij, K, r0)
+            o.init()
+            o.set_field(0, u0)
+            o.step(1)
+            eu = _rel(s.field(), o.field(0))
+            ev = _rel(s.beads()[1], o.beads()[1])
+            print("plane fp32 %s step 1 vs oracle: u %.2e v %.2e" % (dims, eu, ev), flush=True)
+            assert eu < 2e-6 and ev < 2e-6
+            o.close()
+        s.step(2)
+        out[planes] = (s.beads()[1], s.field())
+        s.close()
+    monkeypatch.delenv("SELM_FFT_PLANES", raising=False)
+    ev = _rel(out["1"][0], out["0"][0])
+    eu = _rel(out["1"][1], out["0"][1])
+    print("plane vs transposing fp32 %s: v %.2e u %.2e" % (dims, ev, eu), flush=True)
+    assert ev < 2e-6 and eu < 2e-6
