@@ -110,31 +110,36 @@ __device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint32_t stream, 
     z1 = rad * s;
 }
 
-// The same pair evaluated in FP32 from the same 53-bit integers.  -ln(u1) is
-// taken from u1 when u1 < 1/2 and from log1p(-(1-u1)) otherwise, with 1-u1
-// formed exactly in integer arithmetic, so neither branch loses the low bits
-// of the uniform to rounding.
+// The same pair evaluated in FP32 from the same 53-bit integers, without
+// 64-bit integer arithmetic or branches.  u1 = (a+1) 2^-53 and 1-u1 are both
+// formed from the two 32-bit words with a relative error of at most one FP32
+// rounding each, and -ln(u1) takes one logf: of u1 itself when u1 < 1/2, and
+// of w = RN(1 - x), x = 1-u1, when u1 >= 1/2, with the first-order term of
+// the rounding of w, ((1-w) - x)/w, added back (1-w and that difference are
+// exact by Sterbenz's lemma).  Neither branch of the choice loses the low
+// bits of the uniform, and no lane evaluates two logarithms.
 __device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint32_t stream, uint32_t step,
                                                 uint32_t index, uint32_t slot, float &z0,
                                                 float &z1) {
     uint32_t c[4] = {index, step, slot, stream};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    uint64_t a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
-    uint64_t b = (((uint64_t)c[2] << 32) | c[3]) >> 11;
-    float mlog;
-    if (a + 1 < ((uint64_t)1 << 52)) {
-        mlog = -logf((float)(a + 1) * 0x1p-53f);
-    } else {
-        uint64_t om = ((uint64_t)1 << 53) - (a + 1);  // (1-u1) * 2^53, exact
-        mlog = -log1pf(-(float)om * 0x1p-53f);
-    }
-    int64_t bc = (int64_t)b - ((int64_t)1 << 52);
-    float t = (float)bc * 0x1p-53f;
-    float rad = sqrtf(2.0f * mlog);
-    float s, co;
-    sincospif(2.0f * t, &s, &co);
+    // a = c0 2^21 + (c1 >> 11) in [0, 2^53): u1 = (a + 1) 2^-53
+    const float lo1 = (float)((c[1] >> 11) + 1u);  // exact (< 2^21 + 1)
+    const float u1 = fmaf((float)c[0], 0x1p-32f, lo1 * 0x1p-53f);
+    // 1 - u1 = (2^53 - 1 - a) 2^-53 = ((~c0) 2^21 + (2^21 - 1 - (c1 >> 11))) 2^-53
+    const float lo0 = (float)((0x1FFFFFu - (c[1] >> 11)));  // exact
+    const float x = fmaf((float)(~c[0]), 0x1p-32f, lo0 * 0x1p-53f);
+    const bool small = c[0] < 0x80000000u;  // u1 < 1/2
+    const float w = small ? u1 : 1.0f - x;
+    const float corr = small ? 0.0f : ((1.0f - w) - x) / w;
+    const float mlog = -(logf(w) + corr);
+    // t = (b - 2^52) 2^-53 in [-1/2, 1/2), b = c2 2^21 + (c3 >> 11)
+    const float t = fmaf((float)(int32_t)(c[2] ^ 0x80000000u), 0x1p-32f, (float)(c[3] >> 11) * 0x1p-53f);
+    const float rad = sqrtf(2.0f * mlog);
+    float sn, co;
+    sincospif(2.0f * t, &sn, &co);
     z0 = rad * co;
-    z1 = rad * s;
+    z1 = rad * sn;
 }
 
 template <typename T>

@@ -60,8 +60,11 @@ __device__ __forceinline__ T phi_t(double r, int f4) {
 constexpr int kBeadThreads = 256;
 constexpr int kBeadsPerBlock = kBeadThreads / 16;
 
+// 5 waves per SIMD: 5 workgroups per CU, so 20k beads (1250 workgroups) are
+// resident at once on 256 CUs instead of running in two rounds.
 template <typename T>
-__global__ __launch_bounds__(kBeadThreads) void k_bead_update(const RepDev *__restrict__ reps, int nrep,
+__global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_bead_update(
+    const RepDev *__restrict__ reps, int nrep,
                                                               const long long *__restrict__ start,
                                                               long long total) {
     const int sl = threadIdx.x & 15;  // sub-lane: (li, lj) column of the stencil
@@ -298,10 +301,14 @@ __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T>
             val[j] = (double)(f * w);
         }
         // in bead order: a cell met by several beads of the batch is added
-        // to in the order of the sorted list
+        // to in the order of the sorted list.  LDS atomic adds (ds_add_f64)
+        // from one wave are applied in program order, and the lanes of one
+        // add meet distinct cells, so each cell's sum is the sequential one;
+        // unlike a read-modify-write, no add waits for the previous one.
 #pragma unroll
         for (int j = 0; j < kSpreadBatch; j++)
-            if (cell[j] >= 0) acc[cell[j]] += val[j];
+            if (cell[j] >= 0)
+                __hip_atomic_fetch_add(&acc[cell[j]], val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 
