@@ -188,7 +188,7 @@ constexpr int kTileCells = 512;  // tile edge <= 8
 constexpr int kSpreadChunk = 32;
 constexpr int kSpreadBatch = 8;  // beads whose (cell, value) a lane holds before adding
 constexpr int kSpreadThreads = 256;
-constexpr int kListCap = 2048;
+constexpr int kListCap = 1536;  // with FP32 records the workgroup's LDS stays under 32 KB: 5 per CU
 
 // Records of chunk `base` (nq >= 1 beads of the sorted list) into LDS, by NT
 // threads (lt = 0 .. NT-1) in 16-byte pieces.  Every thread issues all of its
@@ -241,18 +241,10 @@ __device__ __forceinline__ T readlane_t(T v, int k) {
 
 // Wave d adds component d of the chunk's nq beads, in list order, lane =
 // stencil point (li, lj, lk).  A lane's cell on an axis is (p + l) mod n - o;
-// it belongs to the tile if that lies in [0, e).
-//
-// FAST: the offset q = p - o, taken as q - n when q >= e, is the same for the
-// whole wave (a scalar), and each lane only adds l.  This gives the cell
-// above whenever n >= te + 3 on every axis (so n > 8 >= 5):
-// - p in [o, o + e): a wrapped lane (p + l >= n) cannot come back into the
-//   tile, since p + l - n < o + e + 3 - n <= o;
-// - p >= o + e: the stencil meets the tile only through the wrap, where the
-//   cell is p + l - n - o = (q - n) + l;
-// - p < o: a wrapped cell (<= 2) would need o <= 2, while wrapping needs
-//   p >= n - 3, so o > n - 3 > 2.
-template <typename T, bool FAST>
+// it belongs to the tile if that lies in [0, e).  This is the general form,
+// with the wrap taken per lane; spread_add_chunk_fast below covers grids
+// whose edges are all >= tile edge + 3.
+template <typename T>
 __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T> *Q, int nq, int d, int lane,
                                                  const SpreadTile &g) {
     const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
@@ -272,31 +264,18 @@ __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T>
             const SpreadRec<T> &B = Q[min(k, nq - 1)];
             const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
             const T f = readlane_t(fm, k);
-            bool in;
-            int c;
-            if constexpr (FAST) {
-                int qx = __builtin_amdgcn_readlane(px, k) - g.ox;
-                int qy = __builtin_amdgcn_readlane(py, k) - g.oy;
-                int qz = __builtin_amdgcn_readlane(pz, k) - g.oz;
-                qx -= qx >= g.ex ? g.nx : 0;
-                qy -= qy >= g.ey ? g.ny : 0;
-                qz -= qz >= g.ez ? g.nz : 0;
-                in = (unsigned)(qx + li) < (unsigned)g.ex && (unsigned)(qy + lj) < (unsigned)g.ey &&
-                     (unsigned)(qz + lk) < (unsigned)g.ez;
-                c = __builtin_amdgcn_readfirstlane(d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx) + lofs;
-            } else {
-                int cx = __builtin_amdgcn_readlane(px, k) + li;
-                cx -= cx >= g.nx ? g.nx : 0;
-                int cy = __builtin_amdgcn_readlane(py, k) + lj;
-                cy -= cy >= g.ny ? g.ny : 0;
-                int cz = __builtin_amdgcn_readlane(pz, k) + lk;
-                cz -= cz >= g.nz ? g.nz : 0;
-                cx -= g.ox;
-                cy -= g.oy;
-                cz -= g.oz;
-                in = (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
-                c = d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx;
-            }
+            int cx = __builtin_amdgcn_readlane(px, k) + li;
+            cx -= cx >= g.nx ? g.nx : 0;
+            int cy = __builtin_amdgcn_readlane(py, k) + lj;
+            cy -= cy >= g.ny ? g.ny : 0;
+            int cz = __builtin_amdgcn_readlane(pz, k) + lk;
+            cz -= cz >= g.nz ? g.nz : 0;
+            cx -= g.ox;
+            cy -= g.oy;
+            cz -= g.oz;
+            const bool in =
+                (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
+            const int c = d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx;
             cell[j] = in && k < nq ? c : -1;
             val[j] = (double)(f * w);
         }
@@ -305,6 +284,78 @@ __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T>
         // from one wave are applied in program order, and the lanes of one
         // add meet distinct cells, so each cell's sum is the sequential one;
         // unlike a read-modify-write, no add waits for the previous one.
+#pragma unroll
+        for (int j = 0; j < kSpreadBatch; j++)
+            if (cell[j] >= 0)
+                __hip_atomic_fetch_add(&acc[cell[j]], val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// spread_add_chunk for grids whose edges are all >= tile edge + 3.  There
+// the offset q = p - o, taken as q - n when q >= e, is the same for the whole
+// wave (a scalar), and each lane only adds l.  This gives the cell of the
+// general form (so n > 8 >= 5):
+// - p in [o, o + e): a wrapped lane (p + l >= n) cannot come back into the
+//   tile, since p + l - n < o + e + 3 - n <= o;
+// - p >= o + e: the stencil meets the tile only through the wrap, where the
+//   cell is p + l - n - o = (q - n) + l;
+// - p < o: a wrapped cell (<= 2) would need o <= 2, while wrapping needs
+//   p >= n - 3, so o > n - 3 > 2.
+// The per-bead work is moved out of the bead loop: lane m first forms, for
+// bead m of the chunk, the scalar offset of its first stencil cell in the
+// accumulators and the 64-bit mask of its stencil points that lie in the
+// tile (bit 16 lk + 4 lj + li).  The loop then takes the offset, the force
+// and the mask of bead k with readlanes.
+__device__ __forceinline__ unsigned axis_mask4(int q, int e) {
+    unsigned m = 0;
+#pragma unroll
+    for (int l = 0; l < 4; l++) m |= ((unsigned)(q + l) < (unsigned)e ? 1u : 0u) << l;
+    return m;
+}
+
+template <typename T>
+__device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadRec<T> *Q, int nq, int d, int lane,
+                                                      const SpreadTile &g) {
+    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
+    const int lofs = li + g.te0 * (lj + g.te1 * lk);
+    static_assert(kSpreadChunk <= 32, "one bead per lane of the half-wave");
+    const SpreadRec<T> &M = Q[min(lane & 31, nq - 1)];
+    int qx = M.p[d][0] - g.ox, qy = M.p[d][1] - g.oy, qz = M.p[d][2] - g.oz;
+    qx -= qx >= g.ex ? g.nx : 0;
+    qy -= qy >= g.ey ? g.ny : 0;
+    qz -= qz >= g.ez ? g.nz : 0;
+    const int cm = d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx;
+    const unsigned mx = axis_mask4(qx, g.ex), my = axis_mask4(qy, g.ey), mz = axis_mask4(qz, g.ez);
+    uint32_t mlo = 0, mhi = 0;
+#pragma unroll
+    for (int z = 0; z < 4; z++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const uint32_t row = ((mz >> z) & (my >> y) & 1u) ? mx : 0u;
+            if (z < 2) mlo |= row << (16 * z + 4 * y);
+            else mhi |= row << (16 * (z - 2) + 4 * y);
+        }
+    const T fm = M.f[d];
+    const uint32_t lbit = 1u << (lane & 31);
+    for (int k0 = 0; k0 < nq; k0 += kSpreadBatch) {
+        int cell[kSpreadBatch];
+        double val[kSpreadBatch];
+#pragma unroll
+        for (int j = 0; j < kSpreadBatch; j++) {
+            const int k = k0 + j;  // < kSpreadChunk; beads past nq are masked below
+            const SpreadRec<T> &B = Q[min(k, nq - 1)];
+            const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
+            const T f = readlane_t(fm, k);
+            const int c = __builtin_amdgcn_readlane(cm, k);
+            const uint32_t ml = (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
+            const uint32_t mh = (uint32_t)__builtin_amdgcn_readlane((int)mhi, k);
+            const bool in = ((lane < 32 ? ml : mh) & lbit) != 0;
+            cell[j] = in && k < nq ? c + lofs : -1;
+            val[j] = (double)(f * w);
+        }
+        // in bead order: a cell met by several beads of the batch is added
+        // to in the order of the sorted list (LDS atomic adds from one wave
+        // are applied in program order; the lanes of one add meet distinct cells)
 #pragma unroll
         for (int j = 0; j < kSpreadBatch; j++)
             if (cell[j] >= 0)
@@ -382,8 +433,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
             if (nb2 < n) spread_stage<T, 64>(rec[(c + 1) & 1], brec, sorted, nb2, min(kSpreadChunk, n - nb2), lane);
         } else {
             const SpreadTile tl{nx, ny, nz, ox, oy, oz, ex, ey, ez, te0, te1};
-            if (fast) spread_add_chunk<T, true>(acc, rec[c & 1], nq, wv, lane, tl);
-            else spread_add_chunk<T, false>(acc, rec[c & 1], nq, wv, lane, tl);
+            if (fast) spread_add_chunk_fast<T>(acc, rec[c & 1], nq, wv, lane, tl);
+            else spread_add_chunk<T>(acc, rec[c & 1], nq, wv, lane, tl);
         }
         __syncthreads();  // chunk c consumed, chunk c+1 staged
     }
