@@ -492,6 +492,49 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
     const float c_lap = (float)(R.mu * R.dt / (R.rho * R.dx * R.dx)), c_f = (float)(1.0 / R.rho);
     constexpr int BT = IT < 4 ? IT : 4;
     L us[IT];
+    if constexpr (NT % N == 0 && NN % NT == 0) {
+        // Each thread keeps one column x and takes rows y0 + RS k: the column
+        // terms are computed once and every LDS and global address is the
+        // thread's base plus a constant.
+        constexpr int RS = NT / N;
+        const int x = (int)threadIdx.x % N, y0 = (int)threadIdx.x / N;
+        const int xp = x + 1 < N ? x + 1 : 0, xm = x > 0 ? x - 1 : N - 1;
+        const int2 ct = colt[x];
+#pragma unroll
+        for (int k0 = 0; k0 < IT; k0 += BT) {
+            XfLoads<T> v[BT];
+            int ok[BT];
+#pragma unroll
+            for (int j = 0; j < BT; j++) {
+                const int k = k0 + j, y = y0 + RS * k, idx = (int)threadIdx.x + NT * k;
+                const int2 rt = rowt[y];
+                v[j].zp = up[idx];
+                v[j].zm = um[idx];
+                v[j].f = pool[rt.x + ct.x];
+                ok[j] = rt.y + ct.y;
+            }
+#pragma unroll
+            for (int j = 0; j < BT; j++) {
+                const int k = k0 + j, y = y0 + RS * k;
+                const T *row = U + y * N;
+                // rows y +- 1 wrap only for the first and last k
+                const float uyp = k < IT - 1 ? row[N + x] : U[(y + 1 < N ? y + 1 : 0) * N + x];
+                const float uym = k > 0 ? row[x - N] : U[(y > 0 ? y - 1 : N - 1) * N + x];
+                const float uc = row[x];
+                // neighbour order x+, x-, y+, y-, z+, z- (as ustar_at)
+                const float nb = (((((row[xp] + row[xm]) + uyp) + uym) + v[j].zp) + v[j].zm) - 6.0f * uc;
+                const float f = okf[ok[j]] ? v[j].f : 0.0f;
+                us[k] = uc + c_lap * nb + c_f * f;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();  // every thread is done with U
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int y = y0 + RS * k;
+            ((L *)&a[x * S + (y >> 1)])[y & 1] = us[k];
+        }
+    } else {
 #pragma unroll
     for (int k0 = 0; k0 < IT; k0 += BT) {
         // opaque to the optimiser, so that the index arithmetic of a batch is
@@ -537,6 +580,7 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
             const int y = idx / N, x = idx - y * N;
             ((L *)&a[x * S + (y >> 1)])[y & 1] = us[k];
         }
+    }
     }
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };

@@ -337,20 +337,25 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
         }
     const T fm = M.f[d];
     const uint32_t lbit = 1u << (lane & 31);
-    for (int k0 = 0; k0 < nq; k0 += kSpreadBatch) {
+    const bool upper = lane >= 32;
+    const int sink = 3 * kTileCells + lane;  // where a lane outside the tile adds (never read)
+    // fully unrolled over the chunk, so record offsets and readlane lanes are
+    // constants; records past nq are stale and their lanes go to the sink
+#pragma unroll
+    for (int k0 = 0; k0 < kSpreadChunk; k0 += kSpreadBatch) {
+        if (k0 >= nq) break;
         int cell[kSpreadBatch];
         double val[kSpreadBatch];
 #pragma unroll
         for (int j = 0; j < kSpreadBatch; j++) {
-            const int k = k0 + j;  // < kSpreadChunk; beads past nq are masked below
-            const SpreadRec<T> &B = Q[min(k, nq - 1)];
+            const int k = k0 + j;
+            const SpreadRec<T> &B = Q[k];
             const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
             const T f = readlane_t(fm, k);
             const int c = __builtin_amdgcn_readlane(cm, k);
-            const uint32_t ml = (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
-            const uint32_t mh = (uint32_t)__builtin_amdgcn_readlane((int)mhi, k);
-            const bool in = ((lane < 32 ? ml : mh) & lbit) != 0;
-            cell[j] = in && k < nq ? c + lofs : -1;
+            const uint32_t m = upper ? (uint32_t)__builtin_amdgcn_readlane((int)mhi, k)
+                                     : (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
+            cell[j] = (m & lbit) != 0 && k < nq ? c + lofs : sink;
             val[j] = (double)(f * w);
         }
         // in bead order: a cell met by several beads of the batch is added
@@ -358,8 +363,7 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
         // are applied in program order; the lanes of one add meet distinct cells)
 #pragma unroll
         for (int j = 0; j < kSpreadBatch; j++)
-            if (cell[j] >= 0)
-                __hip_atomic_fetch_add(&acc[cell[j]], val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&acc[cell[j]], val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
                                                            long long total) {
     static_assert(3 * kTileCells * sizeof(double) >= kListCap * sizeof(int), "list shares the accumulators");
     __shared__ int sorted[kListCap];
-    __shared__ double acc[3 * kTileCells];  // holds the unsorted list first, then the cell sums
+    __shared__ double acc[3 * kTileCells + 64];  // the unsorted list first, then the cell sums (+ per-lane sinks)
     __shared__ int nlist;
     __shared__ __align__(16) SpreadRec<T> rec[2][kSpreadChunk];  // chunk c in rec[c & 1]
     int *list = (int *)acc;
