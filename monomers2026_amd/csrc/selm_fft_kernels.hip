@@ -633,8 +633,7 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
 // 490 workgroups, within the 512 that two per CU hold at once (16 per
 // column block would make 520, and the last 8 would run after the others).
 // The arithmetic is that of z_body (thermal forcing, projection, k=0
-// removal, 1/N); the forcing of the thread's modes is loaded before the
-// forward transform so its latency is hidden behind it.
+// removal, 1/N).
 constexpr int QZ = 17;
 
 template <typename T, int N>
@@ -660,6 +659,12 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
             const int d = dz / N, e = dz - d * N;
             a[e * S + d * Q + q] = ccast<L>(v);
         });
+    __syncthreads();
+    auto nold = [](int, int) { return cplx<C>{}; };
+    auto nost = [](int, int, cplx<C>) {};
+    fft_run<C, L, N, S, 3 * Q, NT, false, false>(a, tw, -1, nold, nost);
+    // the forcing of this thread's modes, all loads in flight together (held
+    // through the forward transform they would not fit the register budget)
     cplx<T> nv[IP][3];
     if (R.noise) {
         const SELM_GLOBAL cplx<T> *nb = gbl((const cplx<T> *)R.nbuf) + c0;
@@ -671,10 +676,6 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
             for (int d = 0; d < 3; d++) nv[k][d] = ldc(nb + d * R.Nc + (long long)e * pl + q);
         }
     }
-    __syncthreads();
-    auto nold = [](int, int) { return cplx<C>{}; };
-    auto nost = [](int, int, cplx<C>) {};
-    fft_run<C, L, N, S, 3 * Q, NT, false, false>(a, tw, -1, nold, nost);
 
     const C *kxt = (const C *)R.ktabd[0], *kyt = (const C *)R.ktabd[1], *kzt = (const C *)R.ktabd[2];
     const C invN = 1.0 / (double)R.N;
