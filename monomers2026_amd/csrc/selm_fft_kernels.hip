@@ -589,6 +589,28 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
 
     // unpack: element y of sequence s (1 <= s < P) = X_y(s); sequence 0 =
     // X_y(0) + i X_y(P), both real.  Z_p(k) = X_2p(k) + i X_2p+1(k).
+    if constexpr (NT % P == 0 && (N * P) % NT == 0 && (NT / P) % 2 == 0) {
+        // s = tid % P and the parity of y are the same for all of a
+        // thread's elements: the case split is made once per thread
+        constexpr int IT2 = N * P / NT, YS = NT / P;
+        const int s = (int)threadIdx.x % P, y0 = (int)threadIdx.x / P;
+        const bool s0 = s == 0, odd = (y0 & 1) != 0;
+        const int c1 = s, c2 = s0 ? P : N - s;
+        cplx<L> buf[IT2];
+#pragma unroll
+        for (int k = 0; k < IT2; k++) {
+            const int p = (y0 + YS * k) >> 1;
+            const cplx<L> zk = a[c1 * S + p], zm = a[c2 * S + p];
+            // in FP32: a sum of two FP32 values halved is rounded once, as in FP64
+            const L hx = odd ? (zk.y + zm.y) * (L)0.5 : (zk.x + zm.x) * (L)0.5;
+            const L hy = odd ? (zm.x - zk.x) * (L)0.5 : (zk.y - zm.y) * (L)0.5;
+            buf[k] = s0 ? (odd ? mk<L>(zk.y, zm.y) : mk<L>(zk.x, zm.x)) : mk<L>(hx, hy);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT2; k++) a[(y0 + YS * k) * S + s] = buf[k];
+        __syncthreads();
+    } else
     lds_remap<NT, N * P, cplx<L>>(
         [&](int i) {
             const int s = i % P, y = i / P, p = y >> 1;
@@ -767,6 +789,28 @@ __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const 
 
     // pack rows 2p, 2p+1 as Z_e = A_e + i B_e with the Hermitian extension
     // along x; element y of sequence 0 is X_y(0) + i X_y(P)
+    if constexpr (NT % N == 0 && (N * P) % NT == 0) {
+        // e = tid % N is the same for all of a thread's elements: column,
+        // conjugation and the DC/Nyquist case are chosen once per thread
+        constexpr int IT2 = N * P / NT, PS = NT / N;
+        const int e = (int)threadIdx.x % N, p0 = (int)threadIdx.x / N;
+        const bool edge = e == 0 || e == P, nyq = e == P;
+        const int col = edge ? 0 : (e < P ? e : N - e);
+        const L sy = edge ? (L)0 : (e > P ? (L)-1 : (L)1);  // imaginary part kept, conjugated or dropped
+        cplx<L> buf[IT2];
+#pragma unroll
+        for (int k = 0; k < IT2; k++) {
+            const int p = p0 + PS * k;
+            const cplx<L> W0 = a[(2 * p) * S + col], W1 = a[(2 * p + 1) * S + col];
+            const L ax = nyq ? W0.y : W0.x, ay = sy * W0.y;
+            const L bx = nyq ? W1.y : W1.x, by = sy * W1.y;
+            buf[k] = mk<L>(ax - by, ay + bx);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT2; k++) a[e * S + p0 + PS * k] = buf[k];
+        __syncthreads();
+    } else
     lds_remap<NT, N * P, cplx<L>>(
         [&](int i) {
             const int e = i % N, p = i / N;  // consecutive lanes read consecutive elements
