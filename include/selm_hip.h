@@ -161,6 +161,19 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
  *  stages 1-4; here every stage runs alone. */
 int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
 
+/* Autotuner of the FFT pass shapes (no reference counterpart: the
+ * reference's cuFFT plans choose their own, cuda/SELM_Eulerian_Period.cu:26-30).
+ * Times `steps` steps of the system under each shape this build has for its
+ * grid -- FP32-bulk with nx == ny <= 128: the three-pass plane path with 17 or
+ * 9 (kx, ky) columns per Z-pass workgroup, and the five-pass transposing path;
+ * one shape otherwise -- restoring the system's device state after each, and
+ * keeps the fastest.  The choice is cached per (precision, grid shapes) for
+ * the process, and a cached choice is applied without timing.
+ * shape[0] = FFT passes (3, 5; 0 for the runtime-length kernels), shape[1] =
+ * Z-pass columns per workgroup on the plane path (else 0), *ms = the best
+ * milliseconds per step (0 when taken from the cache). */
+int selm_autotune(selm_ctx *ctx, int steps, int shape[2], double *ms);
+
 /* Passes of the 3-D FFT in the ctx's step: 3 (plane path: XY, Z, XY; stages
  * 4 and 6 are then empty), 5 (transposing X, Y, Z, Y, X) or 0 (runtime-length
  * kernels); negative on error.  No reference counterpart: the reference's
@@ -222,6 +235,9 @@ int selm_pool_get_field(selm_pool *pool, int slot, int comp, double *u);
 long long selm_pool_timestep(selm_pool *pool, int slot);
 int selm_pool_retire(selm_pool *pool, int slot);
 /* used/budget bytes of the arena, number of admitted systems */
+/* selm_autotune for a pool: the shapes are timed on the pool's current
+ * systems stepped together, and the pool's device state is restored. */
+int selm_pool_autotune(selm_pool *pool, int steps, int shape[2], double *ms);
 int selm_pool_stats(selm_pool *pool, size_t *used, size_t *budget, int *active);
 void *selm_pool_stream(selm_pool *pool);
 

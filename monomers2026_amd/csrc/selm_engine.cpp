@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -613,6 +614,11 @@ static bool env_is_zero(const char *name) {
     return v && v[0] == '0' && v[1] == 0;
 }
 
+static int env_zq_cols() {
+    const char *v = std::getenv("SELM_ZQ_COLS");
+    return (v && std::strcmp(v, "9") == 0) ? 9 : 17;
+}
+
 // ---------------------------------------------------------------------------
 // A batch of replicas stepped together
 // ---------------------------------------------------------------------------
@@ -621,6 +627,7 @@ struct Batch {
     bool use_graph = true;
     bool force_generic_fft = false;
     bool allow_plane = true;        // three-pass plane FFT path where the shapes allow it
+    int zq_cols = 17;               // plane path: Z-pass columns per workgroup (17 or 9)
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -689,7 +696,7 @@ struct Batch {
         if (plane) {
             for (int r = 0; r < nrep; r++) {
                 size_t a, b;
-                fft_plane_lds(fp64, reps[r]->G.n, twmax, okmax, &a, &b);
+                fft_plane_lds(fp64, reps[r]->G.n, twmax, okmax, zq_cols, &a, &b);
                 lxy = std::max(lxy, a);
                 lzq = std::max(lzq, b);
             }
@@ -714,7 +721,7 @@ struct Batch {
             any_noise = any_noise || R.D.noise;
             if (plane) {
                 long long ixy, iz;
-                fft_plane_items(fp64, G.n, &ixy, &iz);
+                fft_plane_items(fp64, G.n, zq_cols, &ixy, &iz);
                 sfx[r + 1] = sfx[r] + ixy;
                 sfy[r + 1] = sfy[r];
                 sfz[r + 1] = sfz[r] + iz;
@@ -778,6 +785,7 @@ struct Batch {
         L.fast = fast;
         L.plane = plane;
         L.okmax = okmax;
+        L.zq_cols = zq_cols;
         dirty = false;
         return SELM_OK;
     }
@@ -870,6 +878,114 @@ using namespace selm;
 // ===========================================================================
 // selm_ctx: one system
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Autotuning of the FFT pass shapes
+// ---------------------------------------------------------------------------
+// Candidates: the three-pass plane path with 17 or 9 Z-pass columns per
+// workgroup, and the five-pass transposing path.  Each is timed over `steps`
+// graph replays after two warm-up steps; the device memory holding the
+// batch's state (the regions in `mem`) is copied aside first and restored
+// after every candidate, so tuning leaves the systems exactly as it found
+// them.  The fastest shape is cached per (precision, grid shapes) for the
+// process; a cached batch shape is applied without timing.
+struct TuneRegion {
+    void *ptr;
+    size_t bytes;
+};
+
+static std::mutex g_tune_mu;
+static std::map<std::string, std::pair<int, int>> g_tune_cache;  // key -> (passes, zcols)
+
+static std::string tune_key(const Batch &B) {
+    std::string k = B.fp64 ? "fp64" : "fp32";
+    for (const Replica *r : B.reps)
+        k += ":" + std::to_string(r->G.n[0]) + "x" + std::to_string(r->G.n[1]) + "x" + std::to_string(r->G.n[2]);
+    return k;
+}
+
+static void apply_shape(Batch &B, int passes, int zcols) {
+    B.allow_plane = passes == 3;
+    if (zcols) B.zq_cols = zcols;
+    B.dirty = true;
+}
+
+static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int steps, int shape[2], double *ms,
+                          std::string &err) {
+    if (steps < 1) return SELM_ERR_ARG;
+    const std::string key = tune_key(B);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            apply_shape(B, it->second.first, it->second.second);
+            int rc = B.ensure(err);
+            if (rc) return rc;
+            shape[0] = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
+            shape[1] = B.L.plane ? B.L.zq_cols : 0;
+            *ms = 0.0;
+            return SELM_OK;
+        }
+    }
+    hipStream_t st = B.stream;
+    size_t total = 0;
+    for (const auto &m : mem) total += m.bytes;
+    void *save = nullptr;
+    HIPCHK(hipMalloc(&save, std::max<size_t>(total, 256)));
+    size_t off = 0;
+    for (const auto &m : mem) {
+        HIPCHK(hipMemcpyAsync((char *)save + off, m.ptr, m.bytes, hipMemcpyDeviceToDevice, st));
+        off += m.bytes;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    const int cand[3][2] = {{3, 17}, {3, 9}, {5, 0}};
+    int best = -1;
+    double best_ms = 0.0;
+    int rc = SELM_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int c = 0; c < 3 && rc == SELM_OK; c++) {
+        apply_shape(B, cand[c][0], cand[c][1]);
+        rc = B.ensure(err);
+        if (rc) break;
+        const int got = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
+        if (got != cand[c][0] && !(c == 2 && best < 0)) continue;  // shape not available for this batch
+        rc = B.step(2, st, err);
+        if (rc) break;
+        HIPCHK(hipEventRecord(e0, st));
+        rc = B.step(steps, st, err);
+        if (rc) break;
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        const double per = (double)t / steps;
+        if (best < 0 || per < best_ms) {
+            best = c;
+            best_ms = per;
+        }
+        off = 0;
+        for (const auto &m : mem) {
+            HIPCHK(hipMemcpyAsync(m.ptr, (char *)save + off, m.bytes, hipMemcpyDeviceToDevice, st));
+            off += m.bytes;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(save);
+    if (rc) return rc;
+    apply_shape(B, cand[best][0], cand[best][1]);
+    rc = B.ensure(err);
+    if (rc) return rc;
+    shape[0] = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
+    shape[1] = B.L.plane ? B.L.zq_cols : 0;
+    *ms = best_ms;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {cand[best][0], cand[best][1]};
+    return SELM_OK;
+}
+
 struct selm_ctx {
     Batch batch;
     Replica rep;
@@ -919,8 +1035,10 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     c->batch.fp64 = fp64;
     // SELM_FFT_GENERIC=1 selects the runtime-length FFT kernels for every size
     c->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
-    // SELM_FFT_PLANES=0 keeps the five-pass transposing path for every shape
+    // SELM_FFT_PLANES=0 keeps the five-pass transposing path for every shape;
+    // SELM_ZQ_COLS=9 starts the plane path with 9 Z-pass columns per workgroup
     c->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
+    c->batch.zq_cols = env_zq_cols();
     if (c->batch.init_streams(err) != SELM_OK) {
         c->batch.release();
         delete c;
@@ -1118,6 +1236,21 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes) {
     return SELM_OK;
 }
 
+int selm_autotune(selm_ctx *c, int steps, int shape[2], double *ms) {
+    CTX_CHECK(c);
+    if (!shape || !ms) return SELM_ERR_ARG;
+    std::string &err = c->err;
+    if (!c->rep.inited) {
+        err = "selm_autotune before selm_init";
+        return SELM_ERR_STATE;
+    }
+    HIPCHK(hipStreamSynchronize(c->last));
+    c->last = c->batch.stream;
+    std::vector<TuneRegion> mem{{c->grid_mem, c->rep.grid_sz}};
+    if (c->bead_mem && c->rep.bead_sz) mem.push_back({c->bead_mem, c->rep.bead_sz});
+    return autotune_batch(c->batch, mem, steps, shape, ms, err);
+}
+
 int selm_fft_passes(selm_ctx *c) {
     CTX_CHECK(c);
     int rc = c->batch.ensure(c->err);
@@ -1244,6 +1377,7 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->batch.use_graph = pp->use_graph != 0;
     P->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
     P->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
+    P->batch.zq_cols = env_zq_cols();
     if (P->batch.init_streams(P->err) != SELM_OK) {
         P->batch.release();
         delete P;
@@ -1391,6 +1525,19 @@ int selm_pool_retire(selm_pool *P, int slot) {
     P->slots[slot] = nullptr;
     pool_rebuild_list(P);
     return SELM_OK;
+}
+
+int selm_pool_autotune(selm_pool *P, int steps, int shape[2], double *ms) {
+    if (!P || !shape || !ms) return SELM_ERR_ARG;
+    size_t end = 0;
+    for (size_t q = 0; q < P->slots.size(); q++)
+        if (P->slots[q]) end = std::max(end, P->slot_off[q] + P->slot_bytes[q]);
+    if (end == 0) {
+        P->err = "selm_pool_autotune on an empty pool";
+        return SELM_ERR_STATE;
+    }
+    std::vector<TuneRegion> mem{{P->arena, end}};
+    return autotune_batch(P->batch, mem, steps, shape, ms, P->err);
 }
 
 int selm_pool_stats(selm_pool *P, size_t *used, size_t *budget, int *active) {

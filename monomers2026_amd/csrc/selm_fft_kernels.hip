@@ -39,14 +39,14 @@ struct FastCfg<float> {
     using L = float;               // LDS element type
     static constexpr int RB = 32;  // rows per block in Xf/Xi, z per block in Yf/Yi (256 B runs)
     static constexpr int Q = 16;   // (kx, ky) columns per Z workgroup, times 3 components
-    static constexpr int NT = 256, NTZ = 512;
+    static constexpr int NT = 256, NTZ = 512, NTZ9 = 256;
 };
 template <>
 struct FastCfg<double> {
     using L = double;
     static constexpr int RB = 16;
     static constexpr int Q = 8;
-    static constexpr int NT = 256, NTZ = 512;
+    static constexpr int NT = 256, NTZ = 512, NTZ9 = 256;
 };
 
 __device__ __forceinline__ void tw_to_lds(cplx<C> *dst, const void *src, int n) {
@@ -649,20 +649,30 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
     }
 }
 
-// Z pass on the plane layout.  Workgroup: QZ consecutive (kx, ky) columns in
+// Z pass on the plane layout.  Workgroup: Q consecutive (kx, ky) columns in
 // the flat order c = kx * ny + ky, all three components; for each (d, z)
-// the QZ values are consecutive in memory.  With QZ = 17 a 128^3 grid has
-// 490 workgroups, within the 512 that two per CU hold at once (16 per
-// column block would make 520, and the last 8 would run after the others).
-// The arithmetic is that of z_body (thermal forcing, projection, k=0
-// removal, 1/N).
-constexpr int QZ = 17;
+// the Q values are consecutive in memory.  Two shapes are built, chosen per
+// batch (selm_autotune times both): Q = 17 with 512 threads (a 128^3 grid
+// has 490 workgroups, within the 512 that two per CU hold at once; 16
+// columns would make 520, and the last 8 would run after the others) and
+// Q = 9 with 256 threads (925 workgroups, four per CU).  The arithmetic is
+// that of z_body (thermal forcing, projection, k=0 removal, 1/N).
+template <int Q>
+struct ZqShape;
+template <>
+struct ZqShape<17> {
+    static constexpr int NT = 512;
+};
+template <>
+struct ZqShape<9> {
+    static constexpr int NT = 256;
+};
 
-template <typename T, int N>
+template <typename T, int N, int Q>
 __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const cplx<C> *tw,
                                         cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
-    constexpr int Q = QZ, S = 3 * Q + 1, NT = FastCfg<T>::NTZ;
+    constexpr int S = 3 * Q + 1, NT = ZqShape<Q>::NT;
     constexpr int IP = (N * Q + NT - 1) / NT;  // projection elements per thread
     const int ny = R.n[1], xc = R.xc;
     const int pl = xc * ny;  // columns of one (d, z) plane of P
@@ -919,11 +929,14 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
     case n: if constexpr (n <= NMAX) xi_body<T, n>(R, item, tw, a); break;
 
 #define SELM_CASE_ZQ(n) \
-    case n: if constexpr (n <= NMAX) zq_body<T, n>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) zq_body<T, n, 17>(R, item, tw, a); break;
+#define SELM_CASE_ZQ9(n) \
+    case n: if constexpr (n <= NMAX) zq_body<T, n, 9>(R, item, tw, a); break;
 
 SELM_FFT_KERNEL(k_xf, 0, SELM_CASE_XF, NT)
 // 4 waves per SIMD: two 512-thread workgroups per CU
 SELM_FFT_KERNEL(k_zq, 2, SELM_CASE_ZQ, NTZ, __attribute__((amdgpu_waves_per_eu(4))))
+SELM_FFT_KERNEL(k_zq9, 2, SELM_CASE_ZQ9, NTZ9, __attribute__((amdgpu_waves_per_eu(4))))
 
 // XY passes of the plane path.  Dynamic LDS: the x and y twiddles (twmax
 // slots each), the tile stamps of one plane (okmax ints), then the plane.
@@ -1014,11 +1027,11 @@ bool fft_plane_shape(const int n[3]) {
 int fft_plane_okf(const int tn[3]) { return (tn[0] * tn[1] + 3) & ~3; }
 
 // LDS bytes of the XY passes and of the Z pass of the plane path
-void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, size_t *lxy, size_t *lz) {
+void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, int zcols, size_t *lxy, size_t *lz) {
     const size_t csz = fp64 ? sizeof(cplx<FastCfg<double>::L>) : sizeof(cplx<FastCfg<float>::L>);
     const size_t tsz = fp64 ? 8 : 4;
     const size_t N = (size_t)n[0], P = N / 2;
-    const int Q = QZ;
+    const int Q = zcols;
     // twiddles x and y, the row and column pool tables, the tile stamps, the plane
     *lxy = 2 * (size_t)twmax * sizeof(cplx<C>) + 2 * (size_t)twmax * 8 + (size_t)okmax * 4 +
            std::max(N * (P + 1) * csz, N * N * tsz);
@@ -1026,11 +1039,13 @@ void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, size_t *lxy,
 }
 
 // workgroups of the XY passes and of the Z pass for one replica
-void fft_plane_items(bool fp64, const int n[3], long long *ixy, long long *iz) {
+void fft_plane_items(bool fp64, const int n[3], int zcols, long long *ixy, long long *iz) {
     (void)fp64;
     *ixy = 3LL * n[2];
-    *iz = ((long long)(n[0] / 2 + 1) * n[1] + QZ - 1) / QZ;
+    *iz = ((long long)(n[0] / 2 + 1) * n[1] + zcols - 1) / zcols;
 }
+
+bool fft_plane_zcols_ok(int zcols) { return zcols == 17 || zcols == 9; }
 
 // Allow the XY kernels of the size class their dynamic LDS (above 64 KiB).
 template <typename T>
@@ -1061,8 +1076,12 @@ static void launch_plane_n(const StepLaunch &L, int stage, hipStream_t st) {
                                L.st_fx, L.tot_fx, L.twmax, L.okmax);
             break;
         case kStageFftZ:
-            hipLaunchKernelGGL((k_zq<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st, L.reps,
-                               L.nrep, L.st_fz, L.tot_fz, L.twmax);
+            if (L.zq_cols == 9)
+                hipLaunchKernelGGL((k_zq9<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ9), L.lds_fz, st,
+                                   L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
+            else
+                hipLaunchKernelGGL((k_zq<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
+                                   L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
             break;
         case kStageFftXi:
             hipLaunchKernelGGL((k_xyi<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep,

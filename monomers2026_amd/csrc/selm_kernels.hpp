@@ -33,6 +33,7 @@ struct StepLaunch {
     bool fast;                       // compile-time-length FFT kernels (selm_fft_kernels.hip)
     bool plane;                      // ... as the three-pass plane path (XYf, Zq, XYi)
     int okmax;                       // plane path: LDS ints for one plane's tile stamps
+    int zq_cols;                     // plane path: (kx, ky) columns per Z-pass workgroup (17 or 9)
     const long long *st_modes;       // thermal-forcing work table (half-spectrum modes)
     long long tot_modes;             // 0 when no replica has thermal forcing (or not fast)
     hipStream_t st2;                 // forked stream for the thermal forcing
@@ -54,8 +55,9 @@ void launch_fft_fast(const StepLaunch &L, int stage, hipStream_t st);
 void launch_noise(const StepLaunch &L, hipStream_t st);
 bool fft_plane_shape(const int n[3]);
 int fft_plane_okf(const int tn[3]);
-void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, size_t *lxy, size_t *lz);
-void fft_plane_items(bool fp64, const int n[3], long long *ixy, long long *iz);
+void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, int zcols, size_t *lxy, size_t *lz);
+void fft_plane_items(bool fp64, const int n[3], int zcols, long long *ixy, long long *iz);
+bool fft_plane_zcols_ok(int zcols);
 hipError_t fft_plane_prepare(bool fp64, int twmax, size_t lds);
 
 }  // namespace selm

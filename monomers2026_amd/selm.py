@@ -37,7 +37,8 @@ EXPORTS = [
     "selm_pool_create", "selm_pool_destroy", "selm_pool_last_error", "selm_pool_admit",
     "selm_pool_step", "selm_pool_sync", "selm_pool_get_beads", "selm_pool_get_field",
     "selm_pool_timestep", "selm_pool_retire", "selm_pool_stats", "selm_pool_stream",
-    "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes",
+    "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes", "selm_autotune",
+    "selm_pool_autotune",
 ]
 
 
@@ -100,6 +101,8 @@ def load(path=None):
     L.selm_stage_times.argtypes = [vp, C.c_int, dp, C.c_int]
     L.selm_mean_mode.argtypes = [vp, dp]
     L.selm_fft_passes.argtypes = [vp]
+    L.selm_autotune.argtypes = [vp, C.c_int, ip, dp]
+    L.selm_pool_autotune.argtypes = [vp, C.c_int, ip, dp]
     L.selm_stream.argtypes = [vp]
     L.selm_stream.restype = vp
     L.selm_pool_create.argtypes = [C.POINTER(PoolParams), C.POINTER(vp)]
@@ -272,6 +275,14 @@ class System:
         self._chk(self.L.selm_mean_mode(self.h, _d(out)), "mean_mode")
         return out
 
+    def autotune(self, steps=20):
+        """Time the FFT pass shapes on this system and keep the fastest
+        (selm_autotune).  Returns {"passes", "zcols", "ms_per_step"}."""
+        shape = np.zeros(2, np.int32)
+        ms = np.zeros(1)
+        self._chk(self.L.selm_autotune(self.h, int(steps), _i(shape), _d(ms)), "autotune")
+        return {"passes": int(shape[0]), "zcols": int(shape[1]), "ms_per_step": float(ms[0])}
+
     def fft_passes(self):
         """3 (plane path), 5 (transposing path) or 0 (runtime-length kernels)."""
         n = self.L.selm_fft_passes(self.h)
@@ -361,6 +372,14 @@ class Pool:
     def retire(self, slot):
         self._chk(self.L.selm_pool_retire(self.h, slot), "pool_retire")
         self.meta.pop(slot, None)
+
+    def autotune(self, steps=10):
+        """selm_pool_autotune: time the FFT pass shapes on the pool's current
+        systems, restore their state, keep the fastest."""
+        shape = np.zeros(2, np.int32)
+        ms = np.zeros(1)
+        self._chk(self.L.selm_pool_autotune(self.h, int(steps), _i(shape), _d(ms)), "pool_autotune")
+        return {"passes": int(shape[0]), "zcols": int(shape[1]), "ms_per_step": float(ms[0])}
 
     def stats(self):
         used = C.c_size_t(); budget = C.c_size_t(); active = C.c_int()
