@@ -100,17 +100,18 @@ def reference_stage_bytes(w, s):
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this script
 # (--fp32-only --copy-calibration 256), each scaled by what the counter reads
 # for a copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the
-# per-kernel table is profiles/pmc_traffic_dilute20k_128.md.
+# per-kernel table is profiles/pmc_traffic_dilute20k_128.md.  The step runs
+# the plane FFT path (k_xyf, k_zq9, k_xyi): the Y stages have no kernel.
 PMC_TRAFFIC_FP32 = {
-    "thermal_noise": 25835506,
-    "bead_update": 33745288,
-    "spread": 18019950,
-    "ustar_fft_x": 63901715,
-    "fft_y": 51347713,
-    "fft_z_projection": 77068966,
-    "ifft_y": 51335938,
-    "ifft_x": 50921219,
-    "bond_forces": 2709425,
+    "thermal_noise": 25760768,
+    "bead_update": 36836064,
+    "spread": 18076512,
+    "ustar_fft_x": 90185216,
+    "fft_y": 0,
+    "fft_z_projection": 77819776,
+    "ifft_y": 0,
+    "ifft_x": 51052832,
+    "bond_forces": 2709248,
 }
 COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
 
