@@ -856,16 +856,10 @@ __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const 
 }
 
 // ------------------------------------------------------------------ thermal forcing
-// Thread per mode, in the row order of C ([kx][ky][kz], kz fastest).  Runs on
-// a forked stream while the beads are updated and spread.
-// With plane = 1 the order is that of the plane path's P ([kz][kx][ky], ky
-// fastest).
+// One mode of the thermal forcing: its three components into nbuf.
 template <typename T>
-__global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, int nrep,
-                                               const long long *__restrict__ start, long long total,
-                                               int plane) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
+__device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int nrep,
+                                           const long long *__restrict__ start, long long g, int plane) {
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
     if (!R.noise) return;
@@ -891,6 +885,22 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
     SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
 #pragma unroll
     for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
+}
+
+// k_noise: modes in the row order of C ([kx][ky][kz], kz fastest); with plane = 1 in
+// the order of the plane path's P ([kz][kx][ky], ky fastest).  Runs on a
+// forked stream beside the spread and the first FFT pass.  The grid is
+// capped (kNoiseBlocks) and each thread strides over the modes, so the
+// forcing holds a few workgroup slots per CU for longer instead of taking
+// every slot at once while the spread's workgroups wait for them.
+constexpr long long kNoiseBlocks = 1024;  // 4 per CU of the MI355X's 256
+template <typename T>
+__global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, int nrep,
+                                               const long long *__restrict__ start, long long total,
+                                               int plane) {
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x)
+        noise_mode<T>(reps, nrep, start, g, plane);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -1135,7 +1145,7 @@ static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
 
 void launch_noise(const StepLaunch &L, hipStream_t st) {
     if (L.tot_modes <= 0) return;
-    const unsigned nb = (unsigned)((L.tot_modes + 255) / 256);
+    const unsigned nb = (unsigned)std::min<long long>((L.tot_modes + 255) / 256, kNoiseBlocks);
     const int plane = L.plane ? 1 : 0;
     if (L.fp64)
         hipLaunchKernelGGL(k_noise<double>, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_modes, L.tot_modes, plane);

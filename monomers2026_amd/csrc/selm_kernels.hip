@@ -313,18 +313,27 @@ __device__ __forceinline__ unsigned axis_mask4(int q, int e) {
     return m;
 }
 
+// v = bit `lane` of the wave-uniform 64-bit mask ? a : b, as one v_cndmask
+// with the mask in an SGPR pair (a plain C++ select of a mask bit compiles
+// to a shift, an and, a compare and the select).
+__device__ __forceinline__ int lane_select(uint64_t mask, int a, int b) {
+    int v;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(v) : "v"(b), "v"(a), "s"(mask));
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadRec<T> *Q, int nq, int d, int lane,
                                                       const SpreadTile &g) {
     const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
-    const int lofs = li + g.te0 * (lj + g.te1 * lk);
+    const int lofs = 8 * (li + g.te0 * (lj + g.te1 * lk));  // bytes
     static_assert(kSpreadChunk <= 32, "one bead per lane of the half-wave");
     const SpreadRec<T> &M = Q[min(lane & 31, nq - 1)];
     int qx = M.p[d][0] - g.ox, qy = M.p[d][1] - g.oy, qz = M.p[d][2] - g.oz;
     qx -= qx >= g.ex ? g.nx : 0;
     qy -= qy >= g.ey ? g.ny : 0;
     qz -= qz >= g.ez ? g.nz : 0;
-    const int cm = d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx;
+    const int cm = 8 * (d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx);  // bytes
     const unsigned mx = axis_mask4(qx, g.ex), my = axis_mask4(qy, g.ey), mz = axis_mask4(qz, g.ez);
     uint32_t mlo = 0, mhi = 0;
 #pragma unroll
@@ -336,11 +345,11 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
             else mhi |= row << (16 * (z - 2) + 4 * y);
         }
     const T fm = M.f[d];
-    const uint32_t lbit = 1u << (lane & 31);
-    const bool upper = lane >= 32;
-    const int sink = 3 * kTileCells + lane;  // where a lane outside the tile adds (never read)
+    const int sink = 8 * (3 * kTileCells + lane);  // where a lane outside the tile adds (never read)
+    char *accb = (char *)acc;
     // fully unrolled over the chunk, so record offsets and readlane lanes are
-    // constants; records past nq are stale and their lanes go to the sink
+    // constants; the masks of records past nq are cleared, so their lanes go
+    // to the sink
 #pragma unroll
     for (int k0 = 0; k0 < kSpreadChunk; k0 += kSpreadBatch) {
         if (k0 >= nq) break;
@@ -353,9 +362,9 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
             const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
             const T f = readlane_t(fm, k);
             const int c = __builtin_amdgcn_readlane(cm, k);
-            const uint32_t m = upper ? (uint32_t)__builtin_amdgcn_readlane((int)mhi, k)
-                                     : (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
-            cell[j] = (m & lbit) != 0 && k < nq ? c + lofs : sink;
+            const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, k) << 32) |
+                               (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
+            cell[j] = lane_select(k < nq ? m : 0, c + lofs, sink);
             val[j] = (double)(f * w);
         }
         // in bead order: a cell met by several beads of the batch is added
@@ -363,7 +372,8 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
         // are applied in program order; the lanes of one add meet distinct cells)
 #pragma unroll
         for (int j = 0; j < kSpreadBatch; j++)
-            __hip_atomic_fetch_add(&acc[cell[j]], val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add((double *)(accb + cell[j]), val[j], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 
@@ -389,6 +399,10 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
     if (t == 0 && tid == 0) R.ctr[par ^ 1] = 0;  // next step's overflow count (last used a step ago)
     const int cnt = R.bcnt[t];
     if (cnt == 0) return;  // no bead touches this tile: its stamp stays stale
+    // Issue priority over the thermal forcing that runs beside the spread on
+    // the forked stream: the step waits for the spread's fullest tiles, not
+    // for the forcing, which has the first FFT pass to finish in.
+    __builtin_amdgcn_s_setprio(3);
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
     const int ox = tx * R.te[0], oy = ty * R.te[1], oz = tz * R.te[2];
