@@ -103,15 +103,15 @@ def reference_stage_bytes(w, s):
 # per-kernel table is profiles/pmc_traffic_dilute20k_128.md.  The step runs
 # the plane FFT path (k_xyf, k_zq9, k_xyi): the Y stages have no kernel.
 PMC_TRAFFIC_FP32 = {
-    "thermal_noise": 25760768,
-    "bead_update": 36836064,
-    "spread": 18076512,
-    "ustar_fft_x": 90185216,
+    "thermal_noise": 25745408,
+    "bead_update": 36818227,
+    "spread": 18043597,
+    "ustar_fft_x": 90138419,
     "fft_y": 0,
-    "fft_z_projection": 77819776,
+    "fft_z_projection": 77822566,
     "ifft_y": 0,
-    "ifft_x": 51052832,
-    "bond_forces": 2709248,
+    "ifft_x": 51052749,
+    "bond_forces": 2709299,
 }
 COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
 
