@@ -239,3 +239,40 @@ def test_plane_vs_transposing_fft_fp32(gpu_lib, monkeypatch, dims):
     eu = _rel(out["1"][1], out["0"][1])
     print("plane vs transposing fp32 %s: v %.2e u %.2e" % (dims, ev, eu), flush=True)
     assert ev < 2e-6 and eu < 2e-6
+
+
+def test_thermal_forcing_grid_stride(gpu_lib):
+    """A grid with more half-spectrum modes (49 * 96 * 80 = 376,320) than the
+    thermal-forcing launch has threads (1024 workgroups of 256), so each
+    thread draws more than one mode: noise-on steps against the oracle, FP64
+    (five-pass FFT) to 1e-12 and FP32-bulk (plane path) to 1e-6 after two
+    steps (measured: 1.3e-15 and 1.5e-7, the largest in u)."""
+    import oracle_lib as O
+    from monomers2026_amd import selm
+    dims = (96, 96, 80)
+    assert (dims[0] // 2 + 1) * dims[1] * dims[2] > 1024 * 256
+    rng = np.random.default_rng(12)
+    lo, hi = (0.0, 0.0, 0.0), tuple(float(d) for d in dims)
+    M = 60
+    x = rng.uniform(0, min(dims), (M, 3))
+    v = rng.standard_normal((M, 3))
+    m = np.ones(M)
+    ij = np.array([(i, i + 1) for i in range(M - 1)], np.int32)
+    K = np.full(M - 1, 20.0)
+    r0 = np.full(M - 1, 1.0)
+    o = O.Oracle(*dims, lo, hi, rho=1.0, mu=1.0, kB=1.0, T=1.0, dt=0.01, seed=31, noise=True)
+    o.set_beads(x, v, m)
+    o.set_bonds(ij, K, r0)
+    o.init()
+    o.step(2)
+    for prec, tol in ((selm.PREC_FP64, FP64_TOL), (selm.PREC_FP32_BULK, 1e-6)):
+        p = selm.make_params(dims, lo, hi, rho=1.0, mu=1.0, kB=1.0, T=1.0, dt=0.01, seed=31,
+                             stochastic=True, precision=prec)
+        s = selm.System(p)
+        s.set_beads(x, v, m)
+        s.set_bonds(ij, K, r0)
+        s.init()
+        s.step(2)
+        print("grid-stride forcing, precision %d, fft passes %d" % (prec, s.fft_passes()), flush=True)
+        _compare(s, o, tol, "grid-stride forcing %s precision=%d" % (dims, prec), box=float(dims[0]))
+        s.close()
