@@ -219,6 +219,16 @@ __device__ __forceinline__ void spread_stage(SpreadRec<T> *dst, const SpreadRec<
     }
 }
 
+// w[d][0][li] *= f[d] for the nq records of a staged chunk, by NT threads
+// (lt = 0 .. NT-1): the adding waves then take the force with the x factor.
+template <typename T, int NT>
+__device__ __forceinline__ void spread_fold_force(SpreadRec<T> *Q, int nq, int lt) {
+    for (int e = lt; e < nq * 12; e += NT) {
+        const int k = e / 12, r = e - 12 * k, d = r >> 2, li = r & 3;
+        Q[k].w[d][0][li] *= Q[k].f[d];
+    }
+}
+
 struct SpreadTile {
     int nx, ny, nz;  // grid
     int ox, oy, oz;  // tile origin
@@ -254,7 +264,6 @@ __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T>
     static_assert(kSpreadChunk <= 32, "one bead per lane of the half-wave");
     const SpreadRec<T> &M = Q[min(lane & 31, nq - 1)];
     const int px = M.p[d][0], py = M.p[d][1], pz = M.p[d][2];
-    const T fm = M.f[d];
     for (int k0 = 0; k0 < nq; k0 += kSpreadBatch) {
         int cell[kSpreadBatch];
         double val[kSpreadBatch];
@@ -262,8 +271,8 @@ __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T>
         for (int j = 0; j < kSpreadBatch; j++) {
             const int k = k0 + j;  // < kSpreadChunk; beads past nq are masked below
             const SpreadRec<T> &B = Q[min(k, nq - 1)];
+            // the x factor carries the force (spread_fold_force)
             const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
-            const T f = readlane_t(fm, k);
             int cx = __builtin_amdgcn_readlane(px, k) + li;
             cx -= cx >= g.nx ? g.nx : 0;
             int cy = __builtin_amdgcn_readlane(py, k) + lj;
@@ -277,7 +286,7 @@ __device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T>
                 (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
             const int c = d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx;
             cell[j] = in && k < nq ? c : -1;
-            val[j] = (double)(f * w);
+            val[j] = (double)w;
         }
         // in bead order: a cell met by several beads of the batch is added
         // to in the order of the sorted list.  LDS atomic adds (ds_add_f64)
@@ -344,7 +353,6 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
             if (z < 2) mlo |= row << (16 * z + 4 * y);
             else mhi |= row << (16 * (z - 2) + 4 * y);
         }
-    const T fm = M.f[d];
     const int sink = 8 * (3 * kTileCells + lane);  // where a lane outside the tile adds (never read)
     char *accb = (char *)acc;
     // fully unrolled over the chunk, so record offsets and readlane lanes are
@@ -359,13 +367,13 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
         for (int j = 0; j < kSpreadBatch; j++) {
             const int k = k0 + j;
             const SpreadRec<T> &B = Q[k];
+            // the x factor carries the force (spread_fold_force)
             const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
-            const T f = readlane_t(fm, k);
             const int c = __builtin_amdgcn_readlane(cm, k);
             const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, k) << 32) |
                                (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
             cell[j] = lane_select(k < nq ? m : 0, c + lofs, sink);
-            val[j] = (double)(f * w);
+            val[j] = (double)w;
         }
         // in bead order: a cell met by several beads of the batch is added
         // to in the order of the sorted list (LDS atomic adds from one wave
@@ -438,6 +446,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
     spread_stage<T, kSpreadThreads>(rec[0], brec, sorted, 0, min(kSpreadChunk, n), tid);
     for (int c = tid; c < 3 * kTileCells; c += kSpreadThreads) acc[c] = 0.0;
     __syncthreads();
+    spread_fold_force<T, kSpreadThreads>(rec[0], min(kSpreadChunk, n), tid);
+    __syncthreads();
 
     // Waves 0..2 add chunk c (wave d: component d, lane: stencil point) while
     // wave 3 stages chunk c+1 into the other buffer.
@@ -448,7 +458,14 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
         const int nq = min(kSpreadChunk, n - base);
         if (wv == 3) {
             const int nb2 = base + kSpreadChunk;
-            if (nb2 < n) spread_stage<T, 64>(rec[(c + 1) & 1], brec, sorted, nb2, min(kSpreadChunk, n - nb2), lane);
+            if (nb2 < n) {
+                const int nq2 = min(kSpreadChunk, n - nb2);
+                spread_stage<T, 64>(rec[(c + 1) & 1], brec, sorted, nb2, nq2, lane);
+                // the fold reads what other lanes of this wave just stored:
+                // LDS operations of one wave complete in order
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                spread_fold_force<T, 64>(rec[(c + 1) & 1], nq2, lane);
+            }
         } else {
             const SpreadTile tl{nx, ny, nz, ox, oy, oz, ex, ey, ez, te0, te1};
             if (fast) spread_add_chunk_fast<T>(acc, rec[c & 1], nq, wv, lane, tl);
