@@ -355,9 +355,18 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
         }
     const int sink = 8 * (3 * kTileCells + lane);  // where a lane outside the tile adds (never read)
     char *accb = (char *)acc;
-    // fully unrolled over the chunk, so record offsets and readlane lanes are
-    // constants; the masks of records past nq are cleared, so their lanes go
-    // to the sink
+    // The weights of batch b+1 are read before the adds of batch b are
+    // issued: a wave's LDS operations complete in order, so a read issued
+    // after the adds would wait for them.  Records past nq are read (they
+    // lie in the chunk buffer) but their masks are cleared.  The loop is
+    // fully unrolled, so record offsets and readlane lanes are constants.
+    T wz[kSpreadBatch], wy[kSpreadBatch], wx[kSpreadBatch];
+#pragma unroll
+    for (int j = 0; j < kSpreadBatch; j++) {
+        wz[j] = Q[j].w[d][2][lk];
+        wy[j] = Q[j].w[d][1][lj];
+        wx[j] = Q[j].w[d][0][li];  // the x factor carries the force (spread_fold_force)
+    }
 #pragma unroll
     for (int k0 = 0; k0 < kSpreadChunk; k0 += kSpreadBatch) {
         if (k0 >= nq) break;
@@ -366,14 +375,21 @@ __device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadR
 #pragma unroll
         for (int j = 0; j < kSpreadBatch; j++) {
             const int k = k0 + j;
-            const SpreadRec<T> &B = Q[k];
-            // the x factor carries the force (spread_fold_force)
-            const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
+            const T w = (wz[j] * wy[j]) * wx[j];
             const int c = __builtin_amdgcn_readlane(cm, k);
             const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, k) << 32) |
                                (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
             cell[j] = lane_select(k < nq ? m : 0, c + lofs, sink);
             val[j] = (double)w;
+        }
+        if (k0 + kSpreadBatch < kSpreadChunk) {
+#pragma unroll
+            for (int j = 0; j < kSpreadBatch; j++) {
+                const SpreadRec<T> &B = Q[k0 + kSpreadBatch + j];
+                wz[j] = B.w[d][2][lk];
+                wy[j] = B.w[d][1][lj];
+                wx[j] = B.w[d][0][li];
+            }
         }
         // in bead order: a cell met by several beads of the batch is added
         // to in the order of the sorted list (LDS atomic adds from one wave
