@@ -104,16 +104,16 @@ def reference_stage_bytes(w, s):
 # the plane FFT path (k_xyf, k_zq9, k_xyi): the Y stages have no kernel.
 PMC_TRAFFIC_FP32 = {
     "thermal_noise": 25745408,
-    "bead_update": 36818227,
-    "spread": 18043597,
-    "ustar_fft_x": 90138419,
+    "bead_update": 36833312,
+    "spread": 18043648,
+    "ustar_fft_x": 90137344,
     "fft_y": 0,
-    "fft_z_projection": 77822566,
+    "fft_z_projection": 77817840,
     "ifft_y": 0,
-    "ifft_x": 51052749,
-    "bond_forces": 2709299,
+    "ifft_x": 51052832,
+    "bond_forces": 2709248,
 }
-COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
+COPY_CEILING_GBS = 5950.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
 
 # roofline classes: which stages are the spread, the FFT and the interpolation
 CLASSES = {
