@@ -640,12 +640,21 @@ struct Batch {
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // kGraphSteps consecutive steps in one graph: the launch of a graph costs
+    // more than the gap between two kernels inside one
+    static constexpr int kGraphSteps = 4;
+    hipGraph_t graph_n = nullptr;
+    hipGraphExec_t exec_n = nullptr;
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (exec_n) (void)hipGraphExecDestroy(exec_n);
+        if (graph_n) (void)hipGraphDestroy(graph_n);
         exec = nullptr;
         graph = nullptr;
+        exec_n = nullptr;
+        graph_n = nullptr;
     }
 
     int rebuild(std::string &err) {
@@ -818,7 +827,17 @@ struct Batch {
                 graph = g;
                 HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             }
-            for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(exec, st));
+            if (nsteps >= kGraphSteps && !exec_n) {
+                HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                for (int s = 0; s < kGraphSteps; s++) launch_step(L, stream);
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamEndCapture(stream, &g));
+                graph_n = g;
+                HIPCHK(hipGraphInstantiate(&exec_n, graph_n, nullptr, nullptr, 0));
+            }
+            int s = 0;
+            for (; s + kGraphSteps <= nsteps; s += kGraphSteps) HIPCHK(hipGraphLaunch(exec_n, st));
+            for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(exec, st));
         } else {
             for (int s = 0; s < nsteps; s++) launch_step(L, st);
             HIPCHK(hipGetLastError());
