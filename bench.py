@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -121,6 +123,25 @@ CLASSES = {
     "fft": ["ustar_fft_x", "fft_y", "fft_z_projection", "ifft_y", "ifft_x"],
     "interp": ["bead_update"],
 }
+
+
+def unique_stencil_bytes(w, s):
+    """Compulsory stencil bytes: the distinct (component, cell) values that the
+    4x4x4 stencils of all beads touch at the starting positions (box [0, L),
+    dx = 1), times s bytes.  Chains overlap their stencils, so this is the
+    least the gather can read and the spread can write per step."""
+    L = w.edge
+    xs = np.asarray(w.x, np.float64)
+    l = np.arange(4)
+    n = 0
+    for d in range(3):
+        r = xs - np.array([0.5 if a != d else 0.0 for a in range(3)])
+        p = np.floor(r).astype(np.int64) - 1
+        cx = (p[:, 0, None, None, None] + l[None, :, None, None]) % L
+        cy = (p[:, 1, None, None, None] + l[None, None, :, None]) % L
+        cz = (p[:, 2, None, None, None] + l[None, None, None, :]) % L
+        n += np.unique(((cz * L + cy) * L + cx).ravel()).size
+    return n * s
 
 
 def time_steps(sysg, ev, steps, warmup, runs=1):
@@ -350,6 +371,7 @@ def main():
                                        "GBs": round(ref_bytes / (per32 * 1e-3) / 1e9, 1),
                                        "frac": round(ref_bytes / (per32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roof,
+        "unique_stencil_bytes_fp32": unique_stencil_bytes(w, 4),
     }
     if not args.no_ensemble:
         out["ensemble256"] = ensemble_leg(ev)

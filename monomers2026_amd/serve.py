@@ -25,6 +25,7 @@ events recorded on the pool's stream around every step.
 import argparse
 import ctypes as C
 import json
+import sys
 import time
 from collections import deque
 
@@ -198,18 +199,23 @@ def make_requests(rng, duration, rate):
     while t < duration:
         reqs.append(random_request(rng, len(reqs), arrival=t))
         t += rng.exponential(1.0 / rate)
+        if len(reqs) % 1000 == 0:
+            print("generated %d requests" % len(reqs), file=sys.stderr, flush=True)
     return reqs
 
 
-def run(duration=30.0, rate=20.0, budget_gb=16.0, slots=64, seed=5, precision=selm.PREC_FP32_BULK):
+def run(duration=30.0, rate=20.0, budget_gb=16.0, slots=64, seed=5, precision=selm.PREC_FP32_BULK,
+        progress_s=30.0):
     """Replay Poisson arrivals at `rate` requests/s for `duration` seconds of
-    wall clock, then drain.  Returns the report dict."""
+    wall clock, then drain.  Every `progress_s` seconds (0: never) a line of
+    counts goes to stderr.  Returns the report dict."""
     rng = np.random.default_rng(seed)
     reqs = make_requests(rng, duration, rate)
     srv = Server(int(budget_gb * (1 << 30)), slots, precision=precision)
     t0 = time.perf_counter()
     i = 0
     peak = 0
+    next_report = progress_s
     while True:
         now = time.perf_counter() - t0
         while i < len(reqs) and reqs[i].arrival <= now:
@@ -221,6 +227,11 @@ def run(duration=30.0, rate=20.0, budget_gb=16.0, slots=64, seed=5, precision=se
         peak = max(peak, n)
         if n == 0:
             time.sleep(0.0005)
+        if progress_s and now >= next_report:
+            print("t=%.0f s: %d arrived, %d finished, %d running, %d queued, %d batched steps"
+                  % (now, i, len(srv.finished), len(srv.running), len(srv.queue), srv.ticks),
+                  file=sys.stderr, flush=True)
+            next_report += progress_s
     srv.pool.sync()
     wall = time.perf_counter() - t0
     srv.close()
