@@ -54,6 +54,14 @@ struct RepDev {
     double *lj_tab;            // [nt][nt][4]: eps, sigma, rc^2, 0
     int *spec_ptr, *spec_j;    // CSR over beads: 1-2, 1-3, 1-4 partners whose lj weight is not 1
     double *spec_w;
+    // lj/cut neighbour bins (ljb_n = 0: all pairs): ljb_nb[a] >= 3 bins of
+    // edge >= the largest cutoff per axis over [ljb_lo[a], ljb_lo[a] + Lb[a])
+    int ljb_nb[3], ljb_n;
+    double ljb_lo[3];
+    int *ljb_of;               // [M] bin of each bead this step
+    int *ljb_cnt;              // [ljb_n] count, then fill cursor; zero between steps
+    int *ljb_start;            // [ljb_n + 1] first list entry of each bin
+    int *ljb_list;             // [M] bead indices by bin, ascending within a bin
     // ---- scalars
     double dx, lo, hi, L;      // lo[0], hi[0] (reference wraps and grids with dim 0 only)
     double Lb[3];              // per-dimension box lengths (bond minimum image)

@@ -127,7 +127,7 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
 
 // nadj: bond entries (2 per bond), nang: angle entries (3 per angle),
 // nt: lj/cut atom types, nspec: special-partner entries
-static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, int nspec = 0) {
+static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, int nspec = 0, int nbins = 0) {
     size_t b = 0;
     b += 7 * al(3 * (size_t)M * 8);
     b += al((size_t)M * 8);
@@ -138,6 +138,7 @@ static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, i
     b += al(((size_t)M + 1) * 4) + 3 * al((size_t)nang * 4) + 2 * al((size_t)nang * 8);
     b += al((size_t)M * 4) + al((size_t)nt * nt * 4 * 8);
     b += al(((size_t)M + 1) * 4) + al((size_t)nspec * 4) + al((size_t)nspec * 8);
+    b += 2 * al((size_t)M * 4) + al((size_t)nbins * 4) + al(((size_t)nbins + 1) * 4);
     return b;
 }
 
@@ -166,8 +167,9 @@ struct Replica {
     double special_lj[3] = {0, 0, 0};
     std::vector<int> spec_ptr, spec_j;
     std::vector<double> spec_w;
+    int ljb_nb[3] = {0, 0, 0}, ljb_n = 0;  // lj/cut neighbour bins (0: all pairs)
     size_t bytes_beads() const {
-        return bead_bytes(M, (int)adj_i1.size(), fp64, (int)ang_a.size(), lj_nt, (int)spec_j.size());
+        return bead_bytes(M, (int)adj_i1.size(), fp64, (int)ang_a.size(), lj_nt, (int)spec_j.size(), ljb_n);
     }
     bool inited = false;
 };
@@ -248,6 +250,15 @@ static void carve_beads(Replica &R, char *base) {
     R.D.spec_j = (int *)take(nspec * 4);
     R.D.spec_w = (double *)take(nspec * 8);
     R.D.lj_nt = R.lj_nt;
+    R.D.ljb_of = (int *)take((size_t)M * 4);
+    R.D.ljb_list = (int *)take((size_t)M * 4);
+    R.D.ljb_cnt = (int *)take((size_t)R.ljb_n * 4);
+    R.D.ljb_start = (int *)take(((size_t)R.ljb_n + 1) * 4);
+    R.D.ljb_n = R.lj_nt > 0 ? R.ljb_n : 0;
+    for (int a = 0; a < 3; a++) {
+        R.D.ljb_nb[a] = R.ljb_nb[a];
+        R.D.ljb_lo[a] = R.P.lo[a];
+    }
 }
 
 static int replica_setup(Replica &R, const selm_params &p, bool fp64, std::string &err) {
@@ -459,8 +470,27 @@ static int host_set_angles(Replica &R, int na, const int *ijk, const double *K, 
     return SELM_OK;
 }
 
+// Bins for lj/cut: per axis as many as fit with an edge of at least the
+// largest cutoff, at most kLjMaxBins; used when every axis has at least 3
+// (so the 27 bins around a bead are distinct) and `bins` is set.
+constexpr int kLjMaxBins = 64;
+static void lj_bin_geometry(Replica &R, double rcmax, bool bins) {
+    R.ljb_n = 0;
+    for (int a = 0; a < 3; a++) R.ljb_nb[a] = 0;
+    if (!bins || !(rcmax > 0)) return;
+    int n = 1;
+    for (int a = 0; a < 3; a++) {
+        const double La = R.P.hi[a] - R.P.lo[a];
+        const int nb = (int)std::min<double>(kLjMaxBins, std::floor(La / rcmax));
+        if (nb < 3) return;
+        R.ljb_nb[a] = nb;
+        n *= nb;
+    }
+    R.ljb_n = n;
+}
+
 static int host_set_lj(Replica &R, int nt, const int *type, const double *eps, const double *sig, const double *rc,
-                       const double special[3], std::string &err) {
+                       const double special[3], bool bins, std::string &err) {
     if (nt < 0 || (nt > 0 && (!type || !eps || !sig || !rc || !special))) {
         err = "selm_set_pair_lj: bad arguments";
         return SELM_ERR_ARG;
@@ -469,6 +499,7 @@ static int host_set_lj(Replica &R, int nt, const int *type, const double *eps, c
         R.lj_nt = 0;
         R.lj_type.clear();
         R.lj_tab.clear();
+        lj_bin_geometry(R, 0.0, false);
         build_special(R);
         return SELM_OK;
     }
@@ -480,6 +511,7 @@ static int host_set_lj(Replica &R, int nt, const int *type, const double *eps, c
             return SELM_ERR_ARG;
         }
     R.lj_tab.assign((size_t)nt * nt * 4, 0.0);
+    double rcmax = 0.0;
     for (int k = 0; k < nt * nt; k++) {
         if (!(rc[k] >= 0) || 2 * rc[k] >= Lmin) {
             err = "selm_set_pair_lj: cutoff must be >= 0 and below half the box edge";
@@ -488,8 +520,10 @@ static int host_set_lj(Replica &R, int nt, const int *type, const double *eps, c
         R.lj_tab[4 * k] = eps[k];
         R.lj_tab[4 * k + 1] = sig[k];
         R.lj_tab[4 * k + 2] = rc[k] * rc[k];
+        rcmax = std::max(rcmax, rc[k]);
     }
     R.lj_nt = nt;
+    lj_bin_geometry(R, rcmax, bins);
     R.lj_type.assign(type, type + R.M);
     for (int o = 0; o < 3; o++) R.special_lj[o] = special[o];
     build_special(R);
@@ -600,6 +634,7 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
     HIPCHK(hipMemsetAsync(R.D.u, 0, 3 * (size_t)R.G.N * s, st));
     HIPCHK(hipMemsetAsync(R.D.fsp, 0, 3 * (size_t)R.G.N * s, st));
     HIPCHK(hipMemsetAsync(R.D.bcnt, 0, (size_t)R.G.ntiles * 4, st));
+    if (R.D.ljb_n > 0) HIPCHK(hipMemsetAsync(R.D.ljb_cnt, 0, (size_t)R.D.ljb_n * 4, st));
     HIPCHK(hipMemsetAsync(R.D.tstamp, 0xff, (size_t)R.G.ntiles * 4, st));
     HIPCHK(hipMemsetAsync(R.D.ctr, 0, 8 * 4, st));
     HIPCHK(hipMemsetAsync(R.D.ts, 0, 8, st));
@@ -792,6 +827,8 @@ struct Batch {
         L.lds_fz = lfz;
         L.twmax = twmax;
         L.fast = fast;
+        L.lj_bins = false;
+        for (int r = 0; r < nrep; r++) L.lj_bins = L.lj_bins || reps[r]->D.ljb_n > 0;
         L.plane = plane;
         L.okmax = okmax;
         L.zq_cols = zq_cols;
@@ -1143,7 +1180,8 @@ int selm_set_angles(selm_ctx *c, int na, const int *ijk, const double *K, const 
 int selm_set_pair_lj(selm_ctx *c, int ntypes, const int *type, const double *eps, const double *sigma,
                      const double *rc, const double special_lj[3]) {
     CTX_CHECK(c);
-    int r = host_set_lj(c->rep, ntypes, type, eps, sigma, rc, special_lj, c->err);
+    // SELM_LJ_BINS=0 keeps the all-pairs loop (a test switch)
+    int r = host_set_lj(c->rep, ntypes, type, eps, sigma, rc, special_lj, !env_is_zero("SELM_LJ_BINS"), c->err);
     if (r) return r;
     return ctx_alloc_beads(c);
 }

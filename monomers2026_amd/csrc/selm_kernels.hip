@@ -10,12 +10,14 @@
 //                     insertion into the buckets of the tiles its stencils touch
 //   k_spread          workgroup per tile: bucket sorted by bead index in LDS;
 //                     wave d accumulates component d (lanes = stencil points,
-//                     distinct cells, no atomics) and the tile goes to the
-//                     sparse pool; untouched tiles are marked empty
+//                     distinct cells, LDS adds in bead order) and the tile
+//                     goes to the sparse pool; untouched tiles are marked empty
 //   [k_ustar]         u* = u + (mu dt L7 u + f) / rho (runtime-length FFT path
 //                     only; the compile-time-length X pass computes u* itself)
 //   X, Y, Z+projection, Y, X passes of the 3-D FFT (selm_fft_kernels.hip for
 //                     the compile-time lengths, below for the others)
+//   [k_lj_bin, k_lj_scan, k_lj_fill, k_lj_sort]  lj/cut neighbour bins
+//                     (only when a replica has them)
 //   k_bond_forces     thread per bead: harmonic bonds and angles, lj/cut pairs
 //                     and the external force at x^{n+1}; advances each
 //                     replica's step counter
@@ -763,6 +765,118 @@ __global__ __launch_bounds__(256) void k_fft_z_proj(const RepDev *__restrict__ r
 // (E = K (r - r0)^2, F = -2K (r - r0) r_hat, minimum image) in bond order,
 // then fix addforce.  Block 0 also advances every replica's step counter.
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// lj/cut neighbour bins, rebuilt at every force evaluation (beads move):
+// k_lj_bin counts the beads of each bin, k_lj_scan turns the counts into list
+// offsets and fill cursors, k_lj_fill files each bead, and k_lj_sort orders
+// every bin's list by bead index and clears its count for the next step.
+// The lists are thus the same whatever order the atomics ran in.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lj_bin_axis(double x, double lo, double L, int nb) {
+    double s = (x - lo) / L;
+    s -= floor(s);  // periodic image in [0, 1)
+    return min((int)(s * nb), nb - 1);
+}
+
+__global__ __launch_bounds__(256) void k_lj_bin(const RepDev *__restrict__ reps, int nrep,
+                                                const long long *__restrict__ start, long long total) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int r = find_rep(start, nrep, g);
+    const RepDev &R = reps[r];
+    if (R.ljb_n == 0) return;
+    const int b = (int)(g - start[r]);
+    const int ix = lj_bin_axis(R.x[3 * b], R.ljb_lo[0], R.Lb[0], R.ljb_nb[0]);
+    const int iy = lj_bin_axis(R.x[3 * b + 1], R.ljb_lo[1], R.Lb[1], R.ljb_nb[1]);
+    const int iz = lj_bin_axis(R.x[3 * b + 2], R.ljb_lo[2], R.Lb[2], R.ljb_nb[2]);
+    const int bin = (iz * R.ljb_nb[1] + iy) * R.ljb_nb[0] + ix;
+    R.ljb_of[b] = bin;
+    atomicAdd(&R.ljb_cnt[bin], 1);
+}
+
+// one workgroup per replica: exclusive scan of the counts (each thread a
+// contiguous range of bins, then a scan of the 256 range sums)
+__global__ __launch_bounds__(256) void k_lj_scan(const RepDev *__restrict__ reps, int nrep) {
+    const RepDev &R = reps[blockIdx.x];
+    const int n = R.ljb_n;
+    if (n == 0) return;
+    __shared__ int part[256];
+    const int t = threadIdx.x, per = (n + 255) / 256;
+    const int a = min(n, t * per), e = min(n, a + per);
+    int sum = 0;
+    for (int i = a; i < e; i++) sum += R.ljb_cnt[i];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < 256; i++) {
+            const int v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        R.ljb_start[n] = acc;
+    }
+    __syncthreads();
+    int off = part[t];
+    for (int i = a; i < e; i++) {
+        const int c = R.ljb_cnt[i];
+        R.ljb_start[i] = off;
+        R.ljb_cnt[i] = off;  // fill cursor
+        off += c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lj_fill(const RepDev *__restrict__ reps, int nrep,
+                                                 const long long *__restrict__ start, long long total) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int r = find_rep(start, nrep, g);
+    const RepDev &R = reps[r];
+    if (R.ljb_n == 0) return;
+    const int b = (int)(g - start[r]);
+    R.ljb_list[atomicAdd(&R.ljb_cnt[R.ljb_of[b]], 1)] = b;
+}
+
+// one workgroup per replica, a thread per bin: insertion sort of the bin's list
+__global__ __launch_bounds__(256) void k_lj_sort(const RepDev *__restrict__ reps, int nrep) {
+    const RepDev &R = reps[blockIdx.x];
+    for (int i = threadIdx.x; i < R.ljb_n; i += blockDim.x) {
+        const int a = R.ljb_start[i], e = R.ljb_start[i + 1];
+        for (int q = a + 1; q < e; q++) {
+            const int v = R.ljb_list[q];
+            int p = q - 1;
+            while (p >= a && R.ljb_list[p] > v) {
+                R.ljb_list[p + 1] = R.ljb_list[p];
+                p--;
+            }
+            R.ljb_list[p + 1] = v;
+        }
+        R.ljb_cnt[i] = 0;
+    }
+}
+
+// lj/cut force on bead b from bead j (b != j), added to f
+__device__ __forceinline__ void lj_pair(const RepDev &R, int b, int j, int ti, int s0, int s1, double xb, double yb,
+                                        double zb, double &f0, double &f1, double &f2) {
+    double del[3] = {xb - R.x[3 * j], yb - R.x[3 * j + 1], zb - R.x[3 * j + 2]};
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (fabs(del[d]) > 0.5 * R.Lb[d]) del[d] += (del[d] < 0.0) ? R.Lb[d] : -R.Lb[d];
+    const double r2 = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+    const double *tab = R.lj_tab + 4 * (ti * R.lj_nt + R.lj_type[j]);
+    if (r2 >= tab[2]) return;
+    double wgt = 1.0;
+    for (int q = s0; q < s1; q++)
+        if (R.spec_j[q] == j) wgt = R.spec_w[q];
+    if (wgt == 0.0) return;
+    const double s2 = tab[1] * tab[1] / r2;
+    const double s6 = s2 * s2 * s2;
+    const double fr = wgt * 24.0 * tab[0] * (2.0 * s6 * s6 - s6) / r2;
+    f0 += fr * del[0];
+    f1 += fr * del[1];
+    f2 += fr * del[2];
+}
+
 __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ reps, int nrep,
                                                      const long long *__restrict__ start,
                                                      long long total, int advance) {
@@ -844,31 +958,36 @@ __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ 
         }
     }
     // lj/cut (LAMMPS pair_style lj/cut): E = 4 eps ((s/r)^12 - (s/r)^6), r < rc,
-    // minimum image; 1-2/1-3/1-4 partners carry the special weight.  Every
-    // other bead is visited in ascending index.
+    // minimum image; 1-2/1-3/1-4 partners carry the special weight.  With
+    // bins, the 27 bins around the bead's own are visited in a fixed order
+    // (z, y, x offsets -1, 0, 1) and each bin's beads in ascending index; a
+    // bin edge is at least the largest cutoff, so no pair within it is
+    // missed.  Without bins every other bead is visited in ascending index.
     if (R.lj_nt > 0) {
         const int ti = R.lj_type[b];
         const int s0 = R.spec_ptr[b], s1 = R.spec_ptr[b + 1];
         const double xb = R.x[3 * b], yb = R.x[3 * b + 1], zb = R.x[3 * b + 2];
-        for (int j = 0; j < R.M; j++) {
-            if (j == b) continue;
-            double del[3] = {xb - R.x[3 * j], yb - R.x[3 * j + 1], zb - R.x[3 * j + 2]};
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                if (fabs(del[d]) > 0.5 * R.Lb[d]) del[d] += (del[d] < 0.0) ? R.Lb[d] : -R.Lb[d];
-            const double r2 = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
-            const double *tab = R.lj_tab + 4 * (ti * R.lj_nt + R.lj_type[j]);
-            if (r2 >= tab[2]) continue;
-            double wgt = 1.0;
-            for (int q = s0; q < s1; q++)
-                if (R.spec_j[q] == j) wgt = R.spec_w[q];
-            if (wgt == 0.0) continue;
-            const double s2 = tab[1] * tab[1] / r2;
-            const double s6 = s2 * s2 * s2;
-            const double fr = wgt * 24.0 * tab[0] * (2.0 * s6 * s6 - s6) / r2;
-            f0 += fr * del[0];
-            f1 += fr * del[1];
-            f2 += fr * del[2];
+        if (R.ljb_n > 0) {
+            const int nbx = R.ljb_nb[0], nby = R.ljb_nb[1], nbz = R.ljb_nb[2];
+            const int own = R.ljb_of[b];
+            const int ix = own % nbx, iy = (own / nbx) % nby, iz = own / (nbx * nby);
+            for (int dz = -1; dz <= 1; dz++) {
+                const int cz = (iz + dz + nbz) % nbz;
+                for (int dy = -1; dy <= 1; dy++) {
+                    const int cy = (iy + dy + nby) % nby;
+                    for (int dxb = -1; dxb <= 1; dxb++) {
+                        const int cx = (ix + dxb + nbx) % nbx;
+                        const int bin = (cz * nby + cy) * nbx + cx;
+                        for (int q = R.ljb_start[bin]; q < R.ljb_start[bin + 1]; q++) {
+                            const int j = R.ljb_list[q];
+                            if (j != b) lj_pair(R, b, j, ti, s0, s1, xb, yb, zb, f0, f1, f2);
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int j = 0; j < R.M; j++)
+                if (j != b) lj_pair(R, b, j, ti, s0, s1, xb, yb, zb, f0, f1, f2);
         }
     }
     R.fc[3 * b + 0] = f0 + R.fext[3 * b + 0];
@@ -921,8 +1040,7 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
             break;
         case kStageBonds:
             // on the compile-time-length path the Xi pass advances the step counter
-            hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
-                               reps, nrep, L.st_bead, L.tot_bead, L.fast ? 0 : 1);
+            launch_bond_forces(L, L.fast ? 0 : 1, st);
             break;
         default: break;
     }
@@ -984,6 +1102,13 @@ void launch_step(const StepLaunch &L, hipStream_t st) {
 }
 
 void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st) {
+    if (L.lj_bins && L.tot_bead > 0) {
+        const unsigned nb = nblk(L.tot_bead, 256);
+        hipLaunchKernelGGL(k_lj_bin, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_bead, L.tot_bead);
+        hipLaunchKernelGGL(k_lj_scan, dim3(L.nrep), dim3(256), 0, st, L.reps, L.nrep);
+        hipLaunchKernelGGL(k_lj_fill, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_bead, L.tot_bead);
+        hipLaunchKernelGGL(k_lj_sort, dim3(L.nrep), dim3(256), 0, st, L.reps, L.nrep);
+    }
     hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
                        L.reps, L.nrep, L.st_bead, L.tot_bead, advance);
 }

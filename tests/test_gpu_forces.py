@@ -76,3 +76,55 @@ def test_device_forces_steps_match_oracle(gpu_lib):
     assert ex < 1e-11 and ev < 1e-11
     s.close()
     o.close()
+
+
+def _lj_gas(bins, monkeypatch, seed=21):
+    """8000 beads on a jittered lattice in a 24^3 box, chains of 20 with
+    harmonic bonds, two lj/cut types with rc = 2.5 sigma (9 bins per axis)."""
+    from monomers2026_amd import selm
+    if bins:
+        monkeypatch.delenv("SELM_LJ_BINS", raising=False)
+    else:
+        monkeypatch.setenv("SELM_LJ_BINS", "0")
+    rng = np.random.default_rng(seed)
+    L, n = 24.0, 20
+    g = (np.arange(n) + 0.5) * (L / n)
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    x = np.mod(x + rng.uniform(-0.1, 0.1, x.shape), L)
+    M = x.shape[0]
+    ij = np.array([(i, i + 1) for i in range(M - 1) if (i + 1) % 20], np.int32)
+    p = selm.make_params((24, 24, 24), (0.0, 0.0, 0.0), (L, L, L), rho=1.0, mu=1.0, kB=1.0, T=0.5, dt=0.002,
+                         seed=seed, stochastic=True, precision=selm.PREC_FP64)
+    s = selm.System(p)
+    s.set_beads(x, np.zeros_like(x), np.ones(M))
+    s.set_bonds(ij, np.full(len(ij), 20.0), np.full(len(ij), L / n))
+    typ = (np.arange(M) % 3 == 0).astype(np.int32)
+    eps = np.array([[1.0, 0.8], [0.8, 0.6]])
+    sig = np.array([[1.0, 0.9], [0.9, 0.8]])
+    rc = 2.5 * sig
+    s.set_pair_lj(typ, eps, sig, rc, (0.0, 0.0, 0.5))
+    s.init()
+    return s
+
+
+def test_lj_bins_match_all_pairs(gpu_lib, monkeypatch):
+    """lj/cut with neighbour bins against the all-pairs loop (SELM_LJ_BINS=0):
+    the same pairs in another summation order, so forces and a few noise-on
+    steps agree to rounding; two binned runs agree bit for bit."""
+    out = {}
+    for name, bins in (("bins", True), ("pairs", False), ("bins2", True)):
+        s = _lj_gas(bins, monkeypatch)
+        fc0 = s.beads()[2]
+        s.step(10)
+        out[name] = (fc0, s.beads())
+        s.close()
+    monkeypatch.delenv("SELM_LJ_BINS", raising=False)
+    f_b, f_p = out["bins"][0], out["pairs"][0]
+    ef = np.abs(f_b - f_p).max() / np.abs(f_p).max()
+    xb, xp = out["bins"][1][0], out["pairs"][1][0]
+    ex = np.abs(xb - xp).max() / 24.0
+    ev = np.abs(out["bins"][1][1] - out["pairs"][1][1]).max() / np.abs(out["pairs"][1][1]).max()
+    print("lj bins vs all pairs: forces %.2e, after 10 steps x %.2e v %.2e" % (ef, ex, ev), flush=True)
+    assert ef < 1e-12 and ex < 1e-12 and ev < 1e-11
+    for a, b in zip(out["bins"][1], out["bins2"][1]):
+        assert np.array_equal(a, b)
