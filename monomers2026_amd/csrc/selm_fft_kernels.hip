@@ -477,7 +477,17 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
         colt[i] = make_int2(tx * 3 * tcells + (i - tx * te0), tx);
     }
     const SELM_GLOBAL T *uz = ud + (long long)z * NN;
-    copy_batched<NT, NN, (IT < 16 ? IT : 16)>([&](int idx) { return uz[idx]; }, [&](int idx, T v) { U[idx] = v; });
+    // the plane in 16-byte pieces: N is even, so a plane and its offset in u
+    // are whole float4s, and U starts 16-byte aligned in LDS
+    static_assert(NN % 4 == 0, "float4 staging");
+    constexpr int NQ = NN / 4, IQ = (NQ + NT - 1) / NT;
+    {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const SELM_GLOBAL f32x4 *uz4 = (const SELM_GLOBAL f32x4 *)uz;
+        f32x4 *U4 = (f32x4 *)U;
+        copy_batched<NT, NQ, (IQ < 16 ? IQ : 16)>([&](int idx) -> f32x4 { return uz4[idx]; },
+                                                   [&](int idx, f32x4 v) { U4[idx] = v; });
+    }
     __syncthreads();
 
     // u* (the arithmetic of ustar_at, cuda/SELM_Kernel.cu:155-195) of this
