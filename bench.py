@@ -105,17 +105,17 @@ def reference_stage_bytes(w, s):
 # per-kernel table is profiles/pmc_traffic_dilute20k_128.md.  The step runs
 # the plane FFT path (k_xyf, k_zq9, k_xyi): the Y stages have no kernel.
 PMC_TRAFFIC_FP32 = {
-    "thermal_noise": 25745408,
-    "bead_update": 36833312,
-    "spread": 18043648,
-    "ustar_fft_x": 90137344,
+    "thermal_noise": 25744384,
+    "bead_update": 36830848,
+    "spread": 18040480,
+    "ustar_fft_x": 89878656,
     "fft_y": 0,
-    "fft_z_projection": 77817840,
+    "fft_z_projection": 77985920,
     "ifft_y": 0,
-    "ifft_x": 51052832,
-    "bond_forces": 2709248,
+    "ifft_x": 51044640,
+    "bond_forces": 2710272,
 }
-COPY_CEILING_GBS = 5950.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
+COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
 
 # roofline classes: which stages are the spread, the FFT and the interpolation
 CLASSES = {
