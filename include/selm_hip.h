@@ -33,6 +33,17 @@ extern "C" {
 #define SELM_ERR_NOMEM 4        /* device allocation failed */
 #define SELM_ERR_UNSUPPORTED 5  /* grid edge not a product of 2,3,5 in [4, 512] */
 #define SELM_ERR_POOL_FULL 6    /* pool: no slot or not enough budget for this footprint */
+#define SELM_ERR_DEVICE 7       /* a kernel raised a status flag (SELM_STATUS_*); see selm_status */
+
+/* Device status flags of a system, raised by the step's kernels and kept
+ * until cleared with selm_status / selm_pool_status. */
+#define SELM_STATUS_TILE_OVERFLOW 1  /* more than 16384 beads met one 8^3 spread tile: the
+                                        fixed-point cell sums may have wrapped */
+#define SELM_STATUS_NONFINITE 2      /* a bead position or velocity became NaN or infinite */
+#define SELM_STATUS_DISPLACEMENT 4   /* |v dt| > L/2 for some bead component; the reference
+                                        prints this condition and carries on
+                                        (cuda/SELM_Kernel.cu:363-365,
+                                        mpi/SELM_Lagrangian_LAMMPS.cpp:149-151) */
 
 /* precision of the bulk (grid, FFT, spread, interpolation) arithmetic; bead
  * positions, velocities and forces are FP64 in both */
@@ -86,8 +97,19 @@ const char *selm_last_error(const selm_ctx *ctx);
 
 /* Bead state as LAMMPS holds it after read_data (Lagrangian_LAMMPS::initial,
  * cuda/SELM_Lagrangian_LAMMPS.cu:53-69,84-92): x, v AoS [M][3] (v may be NULL
- * for zero velocities, the read_data default), m [M] per-bead mass. Copied in. */
+ * for zero velocities, the read_data default), m [M] per-bead mass. Copied in.
+ * SELM_ERR_ARG for a non-finite x or v or a mass that is not finite and > 0.
+ * selm_init takes positions into the box as LAMMPS's remap does (periodic
+ * image in [lo, hi) per dimension). */
 int selm_set_beads(selm_ctx *ctx, int M, const double *x, const double *v, const double *m);
+
+/* The check selm_set_beads and selm_pool_admit apply to bead inputs, on the
+ * host and without a device (a serving front end can refuse a request before
+ * queueing it): SELM_ERR_ARG and *bad = the first offending bead when a
+ * position or velocity is not finite or a mass is not finite and > 0
+ * (v may be NULL); SELM_OK and *bad = -1 otherwise.  No reference
+ * counterpart: LAMMPS's read_data would accept such values. */
+int selm_check_beads(int M, const double *x, const double *v, const double *m, int *bad);
 
 /* LAMMPS `bond_style harmonic` bonds (Bonds section + bond_coeff), evaluated
  * where the reference calls integrate->setup_minimal
@@ -130,8 +152,17 @@ int selm_init(selm_ctx *ctx);
  * captured hipGraph per step; returns without waiting. */
 int selm_step(selm_ctx *ctx, int nsteps, void *stream);
 
-/* Wait for all work enqueued by this ctx. */
+/* Wait for all work enqueued by this ctx.  Returns SELM_ERR_DEVICE (text in
+ * selm_last_error) when a status flag is raised; the flags stay raised
+ * until selm_status clears them.  selm_step itself only enqueues, so this is
+ * where the host first learns of them. */
 int selm_sync(selm_ctx *ctx);
+
+/* Wait for the ctx's work and return its status flags (SELM_STATUS_*) in
+ * *flags; clear != 0 resets them.  Replaces the reference's printf of a
+ * displacement beyond half the box (cuda/SELM_Kernel.cu:363-365), which the
+ * host never saw. */
+int selm_status(selm_ctx *ctx, int *flags, int clear);
 
 /* Bead state after the last step (waits for it): x, v as in LAMMPS;
  * f_cons = conservative force at the current x (what atom->f holds after
@@ -228,7 +259,12 @@ int selm_pool_admit(selm_pool *pool, const selm_params *p, int M, const double *
 
 /* Advance every admitted system by nsteps steps; returns after enqueueing. */
 int selm_pool_step(selm_pool *pool, int nsteps);
+/* Wait for the pool's work; SELM_ERR_DEVICE when an admitted system has a
+ * status flag raised (the text names the slots). */
 int selm_pool_sync(selm_pool *pool);
+/* Status flags of one admitted system (as selm_status): a serving loop
+ * evicts (retires) a flagged system without disturbing the others. */
+int selm_pool_status(selm_pool *pool, int slot, int *flags, int clear);
 int selm_pool_get_beads(selm_pool *pool, int slot, double *x, double *v, double *f_cons,
                         double *f_total);
 int selm_pool_get_field(selm_pool *pool, int slot, int comp, double *u);

@@ -40,11 +40,14 @@ struct RepDev {
     // ---- beads
     int M;
     double *x, *v, *fc, *ftot, *m, *fext, *tf, *sg;
-    void *brec;                // SpreadRec<T>[M]: per-bead stencil record for the spread
     int *bcnt;                 // per tile: beads inserted this step (may exceed kBucketCap)
-    int *bucket;               // [ntiles][kBucketCap] bead indices, unordered
+    int *slist;                // [ntiles] split tiles of the step (more than kSplitAt bucket
+                               // entries), in the order their count passed kSplitAt
+    double *bucket;            // [ntiles][kBucketCap][6]: s[3] and t_f[3] of the beads inserted
+                               // this step, in insertion order
     int *ovf;                  // [27*M][2] (tile, bead) pairs beyond a full bucket
-    int *ctr;                  // [8]: overflow count per step parity [0,1], status flags [4]
+    int *ctr;                  // [8]: overflow count per step parity [0,1]
+    int *status;               // status flags (kStatus*), one word per replica in the batch's array
     int *adj_ptr, *adj_i1, *adj_i2;   // CSR over beads: bonds touching each bead, bond order
     double *adj_K, *adj_r0;
     int *ang_ptr, *ang_a, *ang_b, *ang_c;  // CSR over beads: harmonic angles (end, centre, end) touching each bead
@@ -76,22 +79,25 @@ struct RepDev {
     double *diag;              // [3] k=0 mode of w (momentum diagnostic), FP64
 };
 
-// Per-bead record written by the bead kernel and read by the spread: for each
-// staggered component d and axis a, the first stencil index (wrapped) and the
-// four 1-D Peskin factors; and -t_f (the impulse the fluid receives).
-template <typename T>
-struct SpreadRec {
-    int p[3][4];
-    T w[3][3][4];
-    T f[4];
-};
-
 // A bead is binned into every tile its stencils touch: on each axis the union
 // of the three staggered stencils spans cells floor(s-1/2)-1 .. floor(s)+2,
 // which meets at most 3 tiles (2 unless a ragged tile is narrower than 4).
 constexpr int kMaxTilesPerBead = 27;
 constexpr int kBucketCap = 256;
-constexpr int kStatusListOverflow = 1;  // a tile held more beads than the spread can order
+constexpr int kSpreadChunk = 32;  // beads a spread wave-group prepares and adds per round
+constexpr int kSplitAt = 64;      // tiles with more bucket entries go to the split-tile spread
+
+// A replica's ctr[8] (zeroed by init), by step parity: [0], [1] the (tile,
+// bead) pairs beyond a full bucket; [4], [5] the split tiles.
+constexpr int kCtrSplits = 4;
+// Status flags (the same values as SELM_STATUS_* in selm_hip.h).
+constexpr int kStatusTileOverflow = 1;   // a tile met more than kMaxTileBeads beads
+constexpr int kStatusNonFinite = 2;      // a bead position or velocity became NaN or infinite
+constexpr int kStatusDisplacement = 4;   // |v dt| > L/2 for a bead component (cuda/SELM_Kernel.cu:363-365)
+// The spread sums each cell as 64-bit fixed point with every term below
+// 2^49 in magnitude, so up to 2^14 terms per cell cannot overflow; a tile
+// whose bead count exceeds this raises kStatusTileOverflow.
+constexpr int kMaxTileBeads = 1 << 14;
 
 // Blocks of a launch are dealt round-robin to the 8 XCDs (each with its own
 // L2); remap so that consecutive logical blocks run on the same XCD and share

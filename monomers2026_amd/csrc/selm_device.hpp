@@ -1,8 +1,8 @@
 // selm_device.hpp -- device-side building blocks for the SELM step on CDNA4.
 //
-// Complex arithmetic, the Philox4x32-10 counter-based generator, the Peskin
-// 4-point 1-D factor, radix-2/3/4/5/8 butterflies and a workgroup-cooperative
-// Stockham FFT over LDS-resident batches of sequences.
+// Complex arithmetic, the Philox4x32-10 counter-based generator,
+// radix-2/3/4/5/8 butterflies and a workgroup-cooperative Stockham FFT over
+// LDS-resident batches of sequences.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -161,17 +161,6 @@ struct Normal<float> {
         normal_pair_f32(seed, stream, step, idx, slot, a, b);
     }
 };
-
-// ---------------------------------------------------------------------------
-// Peskin 4-point 1-D factor.  Branch structure of cuda/SELM_Kernel.cu:387-395:
-// the factor is (3-2r+sqrt(1+4r(1-r)))/8 for r<1, (5-2r-sqrt(-7+4r(3-r)))/8
-// for 1<=r<2, and otherwise stays at 1.0 (f4 != 0, the reference) or 0.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double peskin_phi(double r, int f4) {
-    if (r < 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r * (1.0 - r))) / 8.0;
-    if (r < 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 4.0 * r * (3.0 - r))) / 8.0;
-    return f4 ? 1.0 : 0.0;
-}
 
 __device__ __forceinline__ int wrap_idx(int p, int n) {
     int q = p % n;

@@ -19,6 +19,10 @@
 
 namespace selm {
 
+static_assert(kStatusTileOverflow == SELM_STATUS_TILE_OVERFLOW && kStatusNonFinite == SELM_STATUS_NONFINITE &&
+                  kStatusDisplacement == SELM_STATUS_DISPLACEMENT,
+              "device status flags match the C ABI");
+
 constexpr size_t kLdsMax = 65536;
 constexpr double kPi = 3.14159265358979323846;
 
@@ -42,6 +46,7 @@ struct Geometry {
     long long N, Nc;
     RadixPlan plan[3];
     int te[3], tn[3], ntiles;
+    int gsplit;  // workgroups of the split-tile spread
     int bx, by, bz;
 };
 
@@ -95,6 +100,7 @@ static int make_geometry(const selm_params &p, bool fp64, Geometry &G, std::stri
     G.N = (long long)G.n[0] * G.n[1] * G.n[2];
     G.Nc = (long long)G.xc * G.n[1] * G.n[2];
     G.ntiles = G.tn[0] * G.tn[1] * G.tn[2];
+    G.gsplit = std::max(1, G.ntiles / 16);
     const size_t csz = fp64 ? 16 : 8;
     G.bx = fp64 ? 16 : 32;
     G.by = fp64 ? 8 : 16;
@@ -121,7 +127,8 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     b += al(8) + al(3 * 8);               // ts, diag
     const size_t tcells = (size_t)G.te[0] * G.te[1] * G.te[2];
     b += al((size_t)G.ntiles * 3 * tcells * s);                   // pool
-    b += 2 * al((size_t)G.ntiles * 4) + al((size_t)G.ntiles * kBucketCap * 4) + al(8 * 4);
+    b += 2 * al((size_t)G.ntiles * 4) + al((size_t)G.ntiles * kBucketCap * 48) +
+         al((size_t)G.ntiles * 4) + al(8 * 4);
     return b;
 }
 
@@ -131,7 +138,6 @@ static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, i
     size_t b = 0;
     b += 7 * al(3 * (size_t)M * 8);
     b += al((size_t)M * 8);
-    b += al((size_t)M * spread_rec_bytes(fp64));
     b += al((size_t)kMaxTilesPerBead * M * 8);
     b += al(((size_t)M + 1) * 4);
     b += 2 * al((size_t)nadj * 4) + 2 * al((size_t)nadj * 8);
@@ -208,7 +214,8 @@ static void carve_grid(Replica &R, char *base) {
     R.D.pool = take((size_t)G.ntiles * 3 * tcells * s);
     R.D.tstamp = (int *)take((size_t)G.ntiles * 4);
     R.D.bcnt = (int *)take((size_t)G.ntiles * 4);
-    R.D.bucket = (int *)take((size_t)G.ntiles * kBucketCap * 4);
+    R.D.bucket = (double *)take((size_t)G.ntiles * kBucketCap * 48);
+    R.D.slist = (int *)take((size_t)G.ntiles * 4);
     R.D.ctr = (int *)take(8 * 4);
 }
 
@@ -230,7 +237,6 @@ static void carve_beads(Replica &R, char *base) {
     R.D.tf = (double *)take(3 * (size_t)M * 8);
     R.D.sg = (double *)take(3 * (size_t)M * 8);
     R.D.m = (double *)take((size_t)M * 8);
-    R.D.brec = take((size_t)M * spread_rec_bytes(R.fp64));
     R.D.ovf = (int *)take((size_t)kMaxTilesPerBead * M * 8);
     R.D.adj_ptr = (int *)take(((size_t)M + 1) * 4);
     R.D.adj_i1 = (int *)take(nadj * 4);
@@ -344,17 +350,37 @@ static int upload_tables(Replica &R, hipStream_t st, std::string &err) {
     return SELM_OK;
 }
 
+}  // namespace selm
+
+extern "C" int selm_check_beads(int M, const double *x, const double *v, const double *m, int *bad) {
+    if (bad) *bad = -1;
+    if (M < 0 || (M > 0 && (!x || !m))) return SELM_ERR_ARG;
+    for (int i = 0; i < M; i++) {
+        bool ok = m[i] > 0 && std::isfinite(m[i]);
+        for (int d = 0; d < 3; d++)
+            ok = ok && std::isfinite(x[3 * (size_t)i + d]) && (!v || std::isfinite(v[3 * (size_t)i + d]));
+        if (!ok) {
+            if (bad) *bad = i;
+            return SELM_ERR_ARG;
+        }
+    }
+    return SELM_OK;
+}
+
+namespace selm {
+
 static int host_set_beads(Replica &R, int M, const double *x, const double *v, const double *m,
                           std::string &err) {
     if (M < 0 || (M > 0 && (!x || !m))) {
         err = "selm_set_beads: bad arguments";
         return SELM_ERR_ARG;
     }
-    for (int i = 0; i < M; i++)
-        if (!(m[i] > 0)) {
-            err = "selm_set_beads: bead mass must be > 0";
-            return SELM_ERR_ARG;
-        }
+    int bad = -1;
+    if (selm_check_beads(M, x, v, m, &bad) != SELM_OK) {
+        err = "selm_set_beads: bead " + std::to_string(bad) +
+              " has a non-finite position or velocity, or a mass that is not finite and > 0";
+        return SELM_ERR_ARG;
+    }
     R.M = M;
     R.hx.assign(x, x + 3 * (size_t)M);
     if (v) R.hv.assign(v, v + 3 * (size_t)M);
@@ -577,6 +603,10 @@ static int host_set_external(Replica &R, int n, const int *idx, const double *F,
             err = "selm_set_external_force: bead index out of range";
             return SELM_ERR_ARG;
         }
+        if (!std::isfinite(F[3 * a]) || !std::isfinite(F[3 * a + 1]) || !std::isfinite(F[3 * a + 2])) {
+            err = "selm_set_external_force: non-finite force";
+            return SELM_ERR_ARG;
+        }
         for (int d = 0; d < 3; d++) R.hfext[3 * (size_t)idx[a] + d] += F[3 * a + d];
     }
     return SELM_OK;
@@ -592,6 +622,10 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
             double &xi = xw[3 * (size_t)i + d];
             if (xi < lo) xi += L;
             if (xi >= hi) xi -= L;
+            if (!(xi >= lo && xi < hi)) {  // more than one box length outside
+                xi = lo + ((xi - lo) - L * std::floor((xi - lo) / L));
+                if (xi >= hi) xi -= L;
+            }
             xi = std::max(xi, lo);
         }
     const size_t b3 = 3 * (size_t)R.M * 8;
@@ -637,6 +671,7 @@ static int replica_upload_init(Replica &R, hipStream_t st, std::string &err) {
     if (R.D.ljb_n > 0) HIPCHK(hipMemsetAsync(R.D.ljb_cnt, 0, (size_t)R.D.ljb_n * 4, st));
     HIPCHK(hipMemsetAsync(R.D.tstamp, 0xff, (size_t)R.G.ntiles * 4, st));
     HIPCHK(hipMemsetAsync(R.D.ctr, 0, 8 * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.status, 0, 4, st));
     HIPCHK(hipMemsetAsync(R.D.ts, 0, 8, st));
     HIPCHK(hipMemsetAsync(R.D.diag, 0, 24, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -665,7 +700,7 @@ struct Batch {
     int zq_cols = 17;               // plane path: Z-pass columns per workgroup (17 or 9)
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_split = nullptr;
     std::vector<Replica *> reps;
     RepDev *d_reps = nullptr;
     int cap_reps = 0;
@@ -680,6 +715,10 @@ struct Batch {
     static constexpr int kGraphSteps = 4;
     hipGraph_t graph_n = nullptr;
     hipGraphExec_t exec_n = nullptr;
+    // one status word per system slot (RepDev::status points into it), so
+    // that a batch's flags come back in one copy
+    int *d_status = nullptr;
+    int n_status = 0;
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -692,6 +731,8 @@ struct Batch {
         graph_n = nullptr;
     }
 
+    static constexpr int kTabs = 8;  // work tables: beads, tiles, points, X, Y, Z, modes, split tiles
+
     int rebuild(std::string &err) {
         drop_graph();
         const int nrep = (int)reps.size();
@@ -702,18 +743,18 @@ struct Batch {
             HIPCHK(hipMalloc(&d_reps, sizeof(RepDev) * cap));
             cap_reps = cap;
         }
-        if (7 * (nrep + 1) > cap_tabs) {
+        if (kTabs * (nrep + 1) > cap_tabs) {
             if (d_tabs) (void)hipFree(d_tabs);
             d_tabs = nullptr;
-            int cap = std::max(7 * (nrep + 1), 2 * cap_tabs);
+            int cap = std::max(kTabs * (nrep + 1), 2 * cap_tabs);
             HIPCHK(hipMalloc(&d_tabs, sizeof(long long) * cap));
             cap_tabs = cap;
         }
         std::vector<RepDev> h(nrep);
-        std::vector<long long> tabs(7 * (nrep + 1), 0);
+        std::vector<long long> tabs(kTabs * (nrep + 1), 0);
         long long *sb = tabs.data(), *stl = sb + (nrep + 1), *sen = stl + (nrep + 1);
         long long *sfx = sen + (nrep + 1), *sfy = sfx + (nrep + 1), *sfz = sfy + (nrep + 1);
-        long long *smo = sfz + (nrep + 1);
+        long long *smo = sfz + (nrep + 1), *ssp = smo + (nrep + 1);
         bool any_noise = false;
         const size_t csz = fp64 ? 16 : 8;
         int twmax = 0;
@@ -762,6 +803,7 @@ struct Batch {
             stl[r + 1] = stl[r] + G.ntiles;
             sen[r + 1] = sen[r] + G.N;
             smo[r + 1] = smo[r] + G.Nc;
+            ssp[r + 1] = ssp[r] + G.gsplit;
             any_noise = any_noise || R.D.noise;
             if (plane) {
                 long long ixy, iz;
@@ -812,6 +854,8 @@ struct Batch {
         L.st_fy = d_tabs + 4 * (nrep + 1);
         L.st_fz = d_tabs + 5 * (nrep + 1);
         L.st_modes = d_tabs + 6 * (nrep + 1);
+        L.st_split = d_tabs + 7 * (nrep + 1);
+        L.tot_split = ssp[nrep];
         L.tot_bead = sb[nrep];
         L.tot_tile = stl[nrep];
         L.tot_pts = sen[nrep];
@@ -822,6 +866,7 @@ struct Batch {
         L.st2 = stream2;
         L.ev_fork = ev_fork;
         L.ev_join = ev_join;
+        L.ev_split = ev_split;
         L.lds_fx = lfx;
         L.lds_fy = lfy;
         L.lds_fz = lfz;
@@ -836,12 +881,17 @@ struct Batch {
         return SELM_OK;
     }
 
-    // both streams at the default priority
-    int init_streams(std::string &err) {
+    // both streams at the default priority; nslots status words
+    int init_streams(int nslots, std::string &err) {
+        HIPCHK(hipMalloc(&d_status, sizeof(int) * nslots));
+        HIPCHK(hipMemset(d_status, 0, sizeof(int) * nslots));
+        n_status = nslots;
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_split, hipEventDisableTiming));
+        HIPCHK(spread_split_prepare());
         return SELM_OK;
     }
 
@@ -888,15 +938,27 @@ struct Batch {
         if (d_tabs) (void)hipFree(d_tabs);
         d_reps = nullptr;
         d_tabs = nullptr;
+        if (d_status) (void)hipFree(d_status);
+        d_status = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         if (stream2) (void)hipStreamDestroy(stream2);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_split) (void)hipEventDestroy(ev_split);
         stream = nullptr;
         stream2 = nullptr;
-        ev_fork = ev_join = nullptr;
+        ev_fork = ev_join = ev_split = nullptr;
     }
 };
+
+// Text of a status word (SELM_STATUS_* flags).
+static std::string status_text(int f) {
+    std::string t;
+    if (f & kStatusTileOverflow) t += " tile-overflow(a tile met more than 16384 beads: spread sums out of range)";
+    if (f & kStatusNonFinite) t += " non-finite(a bead position or velocity is NaN or infinite)";
+    if (f & kStatusDisplacement) t += " displacement(|v dt| exceeded half the box length)";
+    return t.empty() ? std::string(" none") : t;
+}
 
 static int download_beads(Replica &R, hipStream_t st, double *x, double *v, double *fc, double *ft,
                           std::string &err) {
@@ -939,7 +1001,8 @@ using namespace selm;
 // ---------------------------------------------------------------------------
 // Candidates: the three-pass plane path with 17 or 9 Z-pass columns per
 // workgroup, and the five-pass transposing path.  Each is timed over `steps`
-// graph replays after two warm-up steps; the device memory holding the
+// graph replays after kGraphSteps + 1 warm-up steps (which instantiate both
+// graphs); the device memory holding the
 // batch's state (the regions in `mem`) is copied aside first and restored
 // after every candidate, so tuning leaves the systems exactly as it found
 // them.  The fastest shape is cached per (precision, grid shapes) for the
@@ -1015,7 +1078,16 @@ static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int step
                 avail[c] = got == cand[c][0] || (c == 2 && !any);  // the last is the fallback
             }
             if (!avail[c]) continue;  // shape not available for this batch
-            rc = B.step(2, st, err);
+            // warm-up long enough to capture and instantiate both graphs
+            // (one step and kGraphSteps steps), so the timed region below
+            // holds graph launches only
+            rc = B.step(Batch::kGraphSteps + 1, st, err);
+            if (rc) break;
+            if (!B.exec || !B.exec_n) {
+                err = "autotune: graphs not instantiated before timing";
+                rc = SELM_ERR_STATE;
+                break;
+            }
             if (rc) break;
             HIPCHK(hipEventRecord(e0, st));
             rc = B.step(steps, st, err);
@@ -1113,7 +1185,7 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     // SELM_ZQ_COLS=9 starts the plane path with 9 Z-pass columns per workgroup
     c->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
     c->batch.zq_cols = env_zq_cols();
-    if (c->batch.init_streams(err) != SELM_OK) {
+    if (c->batch.init_streams(1, err) != SELM_OK) {
         c->batch.release();
         delete c;
         return SELM_ERR_HIP;
@@ -1127,6 +1199,7 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     }
     c->rep.grid_sz = gb;
     carve_grid(c->rep, (char *)c->grid_mem);
+    c->rep.D.status = c->batch.d_status;
     rc = upload_tables(c->rep, c->batch.stream, err);
     if (rc) {
         (void)hipFree(c->grid_mem);
@@ -1263,6 +1336,22 @@ int selm_sync(selm_ctx *c) {
     CTX_CHECK(c);
     std::string &err = c->err;
     HIPCHK(hipStreamSynchronize(c->last));
+    int f = 0;
+    HIPCHK(hipMemcpy(&f, c->batch.d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (f) {
+        err = "device status flags " + std::to_string(f) + ":" + status_text(f);
+        return SELM_ERR_DEVICE;
+    }
+    return SELM_OK;
+}
+
+int selm_status(selm_ctx *c, int *flags, int clear) {
+    CTX_CHECK(c);
+    if (!flags) return SELM_ERR_ARG;
+    std::string &err = c->err;
+    HIPCHK(hipStreamSynchronize(c->last));
+    HIPCHK(hipMemcpy(flags, c->batch.d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (clear) HIPCHK(hipMemset(c->batch.d_status, 0, sizeof(int)));
     return SELM_OK;
 }
 
@@ -1453,7 +1542,7 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
     P->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
     P->batch.zq_cols = env_zq_cols();
-    if (P->batch.init_streams(P->err) != SELM_OK) {
+    if (P->batch.init_streams(pp->max_replicas, P->err) != SELM_OK) {
         P->batch.release();
         delete P;
         return SELM_ERR_HIP;
@@ -1529,6 +1618,7 @@ int selm_pool_admit(selm_pool *P, const selm_params *p, int M, const double *x, 
     HIPCHK(hipStreamSynchronize(P->batch.stream));
     carve_grid(*R, P->arena + off);
     carve_beads(*R, P->arena + off + al(gb));
+    R->D.status = P->batch.d_status + s;
     R->grid_sz = gb;
     R->bead_sz = bb;
     rc = upload_tables(*R, P->batch.stream, err);
@@ -1561,6 +1651,24 @@ int selm_pool_sync(selm_pool *P) {
     if (!P) return SELM_ERR_ARG;
     std::string &err = P->err;
     HIPCHK(hipStreamSynchronize(P->batch.stream));
+    std::vector<int> f(P->slots.size(), 0);
+    HIPCHK(hipMemcpy(f.data(), P->batch.d_status, sizeof(int) * f.size(), hipMemcpyDeviceToHost));
+    std::string bad;
+    for (size_t q = 0; q < f.size(); q++)
+        if (P->slots[q] && f[q]) bad += " slot " + std::to_string(q) + ":" + status_text(f[q]) + ";";
+    if (!bad.empty()) {
+        err = "device status flags on" + bad;
+        return SELM_ERR_DEVICE;
+    }
+    return SELM_OK;
+}
+
+int selm_pool_status(selm_pool *P, int slot, int *flags, int clear) {
+    if (!P || !flags || slot < 0 || slot >= (int)P->slots.size() || !P->slots[slot]) return SELM_ERR_ARG;
+    std::string &err = P->err;
+    HIPCHK(hipStreamSynchronize(P->batch.stream));
+    HIPCHK(hipMemcpy(flags, P->batch.d_status + slot, sizeof(int), hipMemcpyDeviceToHost));
+    if (clear) HIPCHK(hipMemset(P->batch.d_status + slot, 0, sizeof(int)));
     return SELM_OK;
 }
 

@@ -6,12 +6,15 @@
 //
 //   k_bead_update     16 lanes per bead (4 beads per wave): 4x4x4 Peskin
 //                     gather of u^n for the three staggered components, drag +
-//                     thermal kick, x/v/f update, the bead's spread record,
-//                     insertion into the buckets of the tiles its stencils touch
-//   k_spread          workgroup per tile: bucket sorted by bead index in LDS;
-//                     wave d accumulates component d (lanes = stencil points,
-//                     distinct cells, LDS adds in bead order) and the tile
-//                     goes to the sparse pool; untouched tiles are marked empty
+//                     thermal kick, x/v/f update, the bead's grid coordinate and
+//                     impulse, insertion into the buckets of the tiles its
+//                     stencils touch
+//   k_spread          workgroup per tile: the bucket's coordinates and
+//                     impulses staged in LDS, Peskin factors recomputed, every
+//                     (bead, component) added by one wave (lanes = stencil
+//                     points, distinct cells) as 64-bit fixed point, so the
+//                     sums do not depend on the order; the tile goes to the
+//                     sparse pool, untouched tiles are marked empty
 //   [k_ustar]         u* = u + (mu dt L7 u + f) / rho (runtime-length FFT path
 //                     only; the compile-time-length X pass computes u* itself)
 //   X, Y, Z+projection, Y, X passes of the 3-D FFT (selm_fft_kernels.hip for
@@ -25,6 +28,8 @@
 #include "selm_kernels.hpp"
 
 namespace selm {
+
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // Tiles met on one axis by the union of a bead's three staggered stencils.
 __device__ __forceinline__ int axis_tiles(double s, int n, int te, int *out) {
@@ -41,18 +46,34 @@ __device__ __forceinline__ int axis_tiles(double s, int n, int te, int *out) {
     return cnt;
 }
 
-// Peskin factor with the branch decided on the FP64 distance (so the r >= 2
-// quirk and the continuity at r -> 2^- survive) and the arithmetic in T.
+// The four 1-D Peskin factors of a stencil at grid coordinate r: base
+// p0 = floor(r) - 1 and w[l] = phi(|r - (p0 + l)|), phi as in
+// cuda/SELM_Kernel.cu:387-395.  With f = r - floor(r) (exact) the distances
+// are 1+f, f, 1-f and 2-f.  Both branches of phi take the same root there:
+// 1 + 4r(1-r) at r = f or 1-f, and -7 + 4r(3-r) at r = 1+f or 2-f, all equal
+// to 1 + 4f(1-f).  So with s = sqrt(1 + 4f(1-f)):
+//   phi(1+f) = (3 - 2f - s)/8    phi(f)   = (3 - 2f + s)/8
+//   phi(1-f) = (1 + 2f + s)/8    phi(2-f) = (1 + 2f - s)/8
+// (at f = 0, phi(1) = 1/4 by either branch).  At f = 0 the last distance is
+// exactly 2, beyond both branches: the reference leaves the factor at 1.0
+// there (f4 != 0), the textbook kernel has 0.  One root per axis instead of
+// four; the result differs from the branch-by-branch evaluation by rounding.
 template <typename T>
-__device__ __forceinline__ T phi_t(double r, int f4) {
-    if constexpr (sizeof(T) == 8) {
-        return peskin_phi(r, f4);
-    } else {
-        const float q = (float)r;
-        if (r < 1.0) return (3.0f - 2.0f * q + sqrtf(1.0f + 4.0f * q * (1.0f - q))) * 0.125f;
-        if (r < 2.0) return (5.0f - 2.0f * q - sqrtf(fmaxf(-7.0f + 4.0f * q * (3.0f - q), 0.0f))) * 0.125f;
-        return f4 ? 1.0f : 0.0f;
-    }
+__device__ __forceinline__ int peskin4(double r, int f4, T w[4]) {
+    const double fl = floor(r);
+    const T f = (T)(r - fl);
+    const T sq = sqrt((T)1 + (T)4 * f * ((T)1 - f));
+    const T a = (T)3 - (T)2 * f, b = (T)1 + (T)2 * f;
+    w[0] = (a - sq) * (T)0.125;
+    w[1] = (a + sq) * (T)0.125;
+    w[2] = (b + sq) * (T)0.125;
+    w[3] = f == (T)0 ? (f4 ? (T)1 : (T)0) : (b - sq) * (T)0.125;
+    return (int)(fl - 1.0);
+}
+
+template <typename T>
+__device__ __forceinline__ T pick4(const T w[4], int l) {
+    return l == 0 ? w[0] : (l == 1 ? w[1] : (l == 2 ? w[2] : w[3]));
 }
 
 // ===========================================================================
@@ -79,7 +100,6 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const long long N = R.N;
     const int f4 = R.f4;
-    SELM_GLOBAL SpreadRec<T> *rec = gbl((SpreadRec<T> *)R.brec) + b;
     const SELM_GLOBAL double *gx = gbl((const double *)R.x);
 
     double s[3];
@@ -95,60 +115,25 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
         const double rx = s[0] - (d != 0 ? 0.5 : 0.0);
         const double ry = s[1] - (d != 1 ? 0.5 : 0.0);
         const double rz = s[2] - (d != 2 ? 0.5 : 0.0);
-        const int p0 = (int)(floor(rx) - 1.0), p1 = (int)(floor(ry) - 1.0), p2 = (int)(floor(rz) - 1.0);
-        const T fx = phi_t<T>(fabs(rx - (p0 + li)), f4);
-        const T fy = phi_t<T>(fabs(ry - (p1 + lj)), f4);
+        T wx[4], wy[4], fz[4];
+        const int p0 = peskin4<T>(rx, f4, wx), p1 = peskin4<T>(ry, f4, wy), p2 = peskin4<T>(rz, f4, fz);
+        const T fx = pick4(wx, li), fy = pick4(wy, lj);
         const long long col = (long long)wrap_near(p1 + lj, ny) * nx + wrap_near(p0 + li, nx);
         const SELM_GLOBAL T *ud = u + d * N;
-        T fz[4], uv[4];
+        T uv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            fz[k] = phi_t<T>(fabs(rz - (p2 + k)), f4);
-            uv[k] = ud[(long long)wrap_near(p2 + k, nz) * ny * nx + col];
-        }
+        for (int k = 0; k < 4; k++) uv[k] = ud[(long long)wrap_near(p2 + k, nz) * ny * nx + col];
         T acc = (T)0;
 #pragma unroll
         for (int k = 0; k < 4; k++) acc += uv[k] * ((((T)1 * fz[k]) * fy) * fx);
 #pragma unroll
         for (int m = 1; m < 16; m <<= 1) acc += __shfl_xor(acc, m, 64);
         gam[d] = (double)acc;
-        if (active) {
-            if (lj == 0) rec->w[d][0][li] = fx;
-            if (li == 0) rec->w[d][1][lj] = fy;
-            if (sl < 4) rec->w[d][2][sl] = sl == 0 ? fz[0] : (sl == 1 ? fz[1] : (sl == 2 ? fz[2] : fz[3]));
-            if (sl == 0) {
-                rec->p[d][0] = wrap_near(p0, nx);
-                rec->p[d][1] = wrap_near(p1, ny);
-                rec->p[d][2] = wrap_near(p2, nz);
-            }
-        }
     }
     if (!active) return;
 
     const uint32_t step = (uint32_t)(R.ts[0] + 1);
-    {
-        // insert into the bucket of every tile the stencils touch; the bead's
-        // 16 lanes take one (tile) insertion each
-        int tx[6], ty[6], tz[6];
-        const int cx = axis_tiles(s[0], nx, R.te[0], tx);
-        const int cy = axis_tiles(s[1], ny, R.te[1], ty);
-        const int cz = axis_tiles(s[2], nz, R.te[2], tz);
-        const int ncomb = cx * cy * cz;
-        const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
-        for (int q = sl; q < ncomb; q += 16) {
-            const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
-            const int a = q - qa * cx, qb = qa - c * cy;
-            const int t = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
-            const int slot = atomicAdd(&R.bcnt[t], 1);
-            if (slot < kBucketCap) {
-                gbl(R.bucket)[t * kBucketCap + slot] = b;
-            } else {
-                const int o = atomicAdd(&R.ctr[step & 1], 1);
-                gbl(R.ovf)[2 * o] = t;
-                gbl(R.ovf)[2 * o + 1] = b;
-            }
-        }
-    }
+    double tfl = 0.0;  // lanes 0..2 of the bead's group: t_f of component sl
     if (sl < 3) {
         const int d = sl;
         const double gd = d == 0 ? gam[0] : (d == 1 ? gam[1] : gam[2]);
@@ -162,74 +147,90 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
             normal_pair_f64(R.seed, kStreamBead, step, (uint32_t)i, 0u, z0, z1);
             tf += z0 * R.bead_coef;
         }
+        tfl = tf;
         gbl(R.tf)[i] = tf;
         gbl(R.sg)[i] = sd;
-        rec->f[d] = (T)(-tf);
         const double ft = tf + dt * gbl(R.fc)[i];
         gbl(R.ftot)[i] = ft / dt;
         double tmp = vi * dt;
+        // a displacement beyond half the box is the reference's sign of a
+        // broken run (cuda/SELM_Kernel.cu:363-365, where it is printed)
+        int flags = fabs(tmp) > 0.5 * R.L ? kStatusDisplacement : 0;
         tmp += gx[i];
         if (tmp > R.hi) tmp -= R.L;
         else if (tmp < R.lo) tmp += R.L;
+        const double vn = vi + ft / gbl(R.m)[b];
+        flags |= (isfinite(tmp) && isfinite(vn)) ? 0 : kStatusNonFinite;
+        if (flags) atomicOr(R.status, flags);
         gbl(R.x)[i] = tmp;
-        gbl(R.v)[i] = vi + ft / gbl(R.m)[b];
+        gbl(R.v)[i] = vn;
+    }
+    // Insert the bead into the bucket of every tile its stencils touch, the
+    // bead's 16 lanes one (tile) insertion each.  A bucket entry carries what
+    // the spread needs of the bead: s at x^n and t_f.
+    const int g16 = (threadIdx.x & 63) & ~15;
+    const double t0 = __shfl(tfl, g16, 64), t1 = __shfl(tfl, g16 + 1, 64), t2 = __shfl(tfl, g16 + 2, 64);
+    int tx[6], ty[6], tz[6];
+    const int cx = axis_tiles(s[0], nx, R.te[0], tx);
+    const int cy = axis_tiles(s[1], ny, R.te[1], ty);
+    const int cz = axis_tiles(s[2], nz, R.te[2], tz);
+    const int ncomb = cx * cy * cz;
+    const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
+    for (int q = sl; q < ncomb; q += 16) {
+        const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
+        const int a = q - qa * cx, qb = qa - c * cy;
+        const int t = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
+        const int slot = atomicAdd(&R.bcnt[t], 1);
+        if (slot < kBucketCap) {
+            SELM_GLOBAL f64x2 *e = gbl((f64x2 *)R.bucket) + 3 * ((long long)t * kBucketCap + slot);
+            e[0] = f64x2{s[0], s[1]};
+            e[1] = f64x2{s[2], t0};
+            e[2] = f64x2{t1, t2};
+            // a tile of more than kSplitAt entries goes to the split list
+            // (once per step: the count passes kSplitAt once)
+            if (slot == kSplitAt) gbl(R.slist)[atomicAdd(&R.ctr[kCtrSplits + (step & 1)], 1)] = t;
+        } else {
+            const int o = atomicAdd(&R.ctr[step & 1], 1);
+            gbl(R.ovf)[2 * o] = t;
+            gbl(R.ovf)[2 * o + 1] = b;
+        }
     }
 }
 
 // ===========================================================================
 // Spread (Lambda), workgroup per tile.  Semantics of cuda/SELM_Kernel.cu:424-470:
 // each bead adds -t_f * w to the 4x4x4 stencil of each staggered component,
-// w = ((1*phi_z)*phi_y)*phi_x.  Each cell's sum runs over the tile's beads in
-// ascending bead index, so the result does not depend on scheduling or on the
-// other replicas of a batch.
+// w = phi_z phi_y phi_x, at the grid coordinate s = (x^n - lo)/dx of the
+// bead update.
+//
+// Each cell is summed in 64-bit fixed point.  Every term -t_f w is rounded
+// once to a multiple of 2^-S, with S = 1071 - E and E the largest biased
+// FP64 exponent of the impulses t_f of the tile's beads (so |term| 2^S <=
+// 2^49); integer sums do not depend on the order of the additions.  The
+// tile's beads are therefore taken in whatever order the bucket insertions
+// left them, with no sort, all four waves add concurrently (ds_add_u64), and
+// the result is bitwise the same from run to run and alone or in a batch.
+// The tile is converted to T once when it is written.
+//
+// Per bead the kernel reads s and t_f (48 bytes) and recomputes the Peskin
+// factors with the bead update's own arithmetic (peskin4 on the same FP64
+// coordinate), so the factors are bitwise those of the gather.
 // ===========================================================================
-// The cell sums are FP64 in both precisions: each product -t_f * w is formed
-// in T and added to an FP64 accumulator in LDS, and the tile is rounded to T
-// once when it is written.
-constexpr int kTileCells = 512;  // tile edge <= 8
-constexpr int kSpreadChunk = 32;
-constexpr int kSpreadBatch = 8;  // beads whose (cell, value) a lane holds before adding
+constexpr int kTileCells = 512;   // tile edge <= 8
 constexpr int kSpreadThreads = 256;
-constexpr int kListCap = 1536;  // with FP32 records the workgroup's LDS stays under 32 KB: 5 per CU
 
-// Records of chunk `base` (nq >= 1 beads of the sorted list) into LDS, by NT
-// threads (lt = 0 .. NT-1) in 16-byte pieces.  Every thread issues all of its
-// loads (indices clamped into the chunk) before its first store, so a chunk
-// costs about one global-load latency.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-template <typename T, int NT>
-__device__ __forceinline__ void spread_stage(SpreadRec<T> *dst, const SpreadRec<T> *__restrict__ brec,
-                                             const int *sorted, int base, int nq, int lt) {
-    static_assert(sizeof(SpreadRec<T>) % 16 == 0, "records are whole 16-byte pieces");
-    constexpr int V = sizeof(SpreadRec<T>) / 16;
-    constexpr int PER = (kSpreadChunk * V + NT - 1) / NT;
-    const int tot = nq * V;
-    u32x4 v[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int idx = min(lt + q * NT, tot - 1);
-        const int k = idx / V;
-        v[q] = gbl((const u32x4 *)&brec[sorted[base + k]])[idx - k * V];
-    }
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int idx = lt + q * NT;
-        if (idx < tot) {
-            const int k = idx / V;
-            ((u32x4 *)&dst[k])[idx - k * V] = v[q];
-        }
-    }
+// round(v 2^S) as a 64-bit integer for |v 2^S| < 2^51 (scale = 2^S): adding
+// 1.5 2^52 leaves the rounded integer in the low mantissa bits.
+__device__ __forceinline__ unsigned long long fixed_term(double v, double scale) {
+    const double t = __builtin_fma(v, scale, 6755399441055744.0);
+    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
 }
 
-// w[d][0][li] *= f[d] for the nq records of a staged chunk, by NT threads
-// (lt = 0 .. NT-1): the adding waves then take the force with the x factor.
-template <typename T, int NT>
-__device__ __forceinline__ void spread_fold_force(SpreadRec<T> *Q, int nq, int lt) {
-    for (int e = lt; e < nq * 12; e += NT) {
-        const int k = e / 12, r = e - 12 * k, d = r >> 2, li = r & 3;
-        Q[k].w[d][0][li] *= Q[k].f[d];
-    }
-}
+// biased FP64 exponent of |v| (|v| < 2^(e - 1022) for e >= 1; 0 for v = 0)
+__device__ __forceinline__ int exp_bits(double v) { return (int)((__double_as_longlong(v) >> 52) & 0x7ff); }
+
+// scale exponent S for a largest biased impulse exponent E
+__device__ __forceinline__ int spread_shift(int E) { return max(-1000, min(1000, 1071 - E)); }
 
 struct SpreadTile {
     int nx, ny, nz;  // grid
@@ -238,85 +239,26 @@ struct SpreadTile {
     int te0, te1;    // tile edge (cell index strides)
 };
 
-// v of lane k (k uniform), as a scalar
-template <typename T>
-__device__ __forceinline__ T readlane_t(T v, int k) {
-    if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-    } else {
-        const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
-        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-    }
-}
-
-// Wave d adds component d of the chunk's nq beads, in list order, lane =
-// stencil point (li, lj, lk).  A lane's cell on an axis is (p + l) mod n - o;
-// it belongs to the tile if that lies in [0, e).  This is the general form,
-// with the wrap taken per lane; spread_add_chunk_fast below covers grids
-// whose edges are all >= tile edge + 3.
-template <typename T>
-__device__ __forceinline__ void spread_add_chunk(double *acc, const SpreadRec<T> *Q, int nq, int d, int lane,
-                                                 const SpreadTile &g) {
-    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
-    const int lofs = li + g.te0 * (lj + g.te1 * lk);
-    // lane m (< 32) holds bead m's stencil base and force for this component;
-    // the loop takes them per bead with readlane
-    static_assert(kSpreadChunk <= 32, "one bead per lane of the half-wave");
-    const SpreadRec<T> &M = Q[min(lane & 31, nq - 1)];
-    const int px = M.p[d][0], py = M.p[d][1], pz = M.p[d][2];
-    for (int k0 = 0; k0 < nq; k0 += kSpreadBatch) {
-        int cell[kSpreadBatch];
-        double val[kSpreadBatch];
-#pragma unroll
-        for (int j = 0; j < kSpreadBatch; j++) {
-            const int k = k0 + j;  // < kSpreadChunk; beads past nq are masked below
-            const SpreadRec<T> &B = Q[min(k, nq - 1)];
-            // the x factor carries the force (spread_fold_force)
-            const T w = (B.w[d][2][lk] * B.w[d][1][lj]) * B.w[d][0][li];
-            int cx = __builtin_amdgcn_readlane(px, k) + li;
-            cx -= cx >= g.nx ? g.nx : 0;
-            int cy = __builtin_amdgcn_readlane(py, k) + lj;
-            cy -= cy >= g.ny ? g.ny : 0;
-            int cz = __builtin_amdgcn_readlane(pz, k) + lk;
-            cz -= cz >= g.nz ? g.nz : 0;
-            cx -= g.ox;
-            cy -= g.oy;
-            cz -= g.oz;
-            const bool in =
-                (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
-            const int c = d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx;
-            cell[j] = in && k < nq ? c : -1;
-            val[j] = (double)w;
-        }
-        // in bead order: a cell met by several beads of the batch is added
-        // to in the order of the sorted list.  LDS atomic adds (ds_add_f64)
-        // from one wave are applied in program order, and the lanes of one
-        // add meet distinct cells, so each cell's sum is the sequential one;
-        // unlike a read-modify-write, no add waits for the previous one.
-#pragma unroll
-        for (int j = 0; j < kSpreadBatch; j++)
-            if (cell[j] >= 0)
-                __hip_atomic_fetch_add(&acc[cell[j]], val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-}
-
-// spread_add_chunk for grids whose edges are all >= tile edge + 3.  There
-// the offset q = p - o, taken as q - n when q >= e, is the same for the whole
-// wave (a scalar), and each lane only adds l.  This gives the cell of the
-// general form (so n > 8 >= 5):
+// Per (bead, component) unit u = 3k + d of a staged chunk.  On grids whose
+// edges are all >= tile edge + 3 ("fast"): a = byte offset in the
+// accumulators of the stencil's first cell, mask = the stencil points (bit
+// 16 lk + 4 lj + li) that lie in the tile.  There the offset q = p - o,
+// taken as q - n when q >= e, is the same for the whole stencil, and each
+// lane only adds its l; this gives the cell (p + l) mod n - o of the general
+// form (so n > 8 >= 5):
 // - p in [o, o + e): a wrapped lane (p + l >= n) cannot come back into the
 //   tile, since p + l - n < o + e + 3 - n <= o;
 // - p >= o + e: the stencil meets the tile only through the wrap, where the
 //   cell is p + l - n - o = (q - n) + l;
 // - p < o: a wrapped cell (<= 2) would need o <= 2, while wrapping needs
 //   p >= n - 3, so o > n - 3 > 2.
-// The per-bead work is moved out of the bead loop: lane m first forms, for
-// bead m of the chunk, the scalar offset of its first stencil cell in the
-// accumulators and the 64-bit mask of its stencil points that lie in the
-// tile (bit 16 lk + 4 lj + li).  The loop then takes the offset, the force
-// and the mask of bead k with readlanes.
+// Otherwise ("general"): a, b, mask = the wrapped stencil base p per axis,
+// and each lane wraps its own cell.
+struct SpreadUnit {
+    int a, b;
+    uint64_t mask;
+};
+
 __device__ __forceinline__ unsigned axis_mask4(int q, int e) {
     unsigned m = 0;
 #pragma unroll
@@ -333,171 +275,304 @@ __device__ __forceinline__ int lane_select(uint64_t mask, int a, int b) {
     return v;
 }
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Unit u = 3k + d of a staged chunk (thread u < 3 nq; entry k holds s[3]
+// and t_f[3]): the 12 Peskin factors w[u][axis][l] of component d at the
+// bead's coordinate s (x factors times -t_f) and the unit's addressing.
 template <typename T>
-__device__ __forceinline__ void spread_add_chunk_fast(double *acc, const SpreadRec<T> *Q, int nq, int d, int lane,
-                                                      const SpreadTile &g) {
-    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
-    const int lofs = 8 * (li + g.te0 * (lj + g.te1 * lk));  // bytes
-    static_assert(kSpreadChunk <= 32, "one bead per lane of the half-wave");
-    const SpreadRec<T> &M = Q[min(lane & 31, nq - 1)];
-    int qx = M.p[d][0] - g.ox, qy = M.p[d][1] - g.oy, qz = M.p[d][2] - g.oz;
-    qx -= qx >= g.ex ? g.nx : 0;
-    qy -= qy >= g.ey ? g.ny : 0;
-    qz -= qz >= g.ez ? g.nz : 0;
-    const int cm = 8 * (d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx);  // bytes
-    const unsigned mx = axis_mask4(qx, g.ex), my = axis_mask4(qy, g.ey), mz = axis_mask4(qz, g.ez);
-    uint32_t mlo = 0, mhi = 0;
+__device__ __forceinline__ void spread_prepare(const double *ent, T *w, SpreadUnit *units, int u, int f4,
+                                               const SpreadTile &g, bool fast) {
+    const int k = u / 3, d = u - 3 * k;
+    const double *sg = ent + 6 * k;
+    const double fd = -sg[3 + d];
+    const int n[3] = {g.nx, g.ny, g.nz};
+    int p[3];
 #pragma unroll
-    for (int z = 0; z < 4; z++)
+    for (int a = 0; a < 3; a++) {
+        // component d is sampled at its face centre: the other two axes shift by -1/2
+        const double r = sg[a] - (d != a ? 0.5 : 0.0);
+        T f[4];
+        const int p0 = peskin4<T>(r, f4, f);
 #pragma unroll
-        for (int y = 0; y < 4; y++) {
-            const uint32_t row = ((mz >> z) & (my >> y) & 1u) ? mx : 0u;
-            if (z < 2) mlo |= row << (16 * z + 4 * y);
-            else mhi |= row << (16 * (z - 2) + 4 * y);
-        }
-    const int sink = 8 * (3 * kTileCells + lane);  // where a lane outside the tile adds (never read)
-    char *accb = (char *)acc;
-    // The weights of batch b+1 are read before the adds of batch b are
-    // issued: a wave's LDS operations complete in order, so a read issued
-    // after the adds would wait for them.  Records past nq are read (they
-    // lie in the chunk buffer) but their masks are cleared.  The loop is
-    // fully unrolled, so record offsets and readlane lanes are constants.
-    T wz[kSpreadBatch], wy[kSpreadBatch], wx[kSpreadBatch];
-#pragma unroll
-    for (int j = 0; j < kSpreadBatch; j++) {
-        wz[j] = Q[j].w[d][2][lk];
-        wy[j] = Q[j].w[d][1][lj];
-        wx[j] = Q[j].w[d][0][li];  // the x factor carries the force (spread_fold_force)
+        for (int l = 0; l < 4; l++) w[12 * u + 4 * a + l] = a == 0 ? f[l] * (T)fd : f[l];  // x carries the force
+        p[a] = wrap_near(p0, n[a]);
     }
+    SpreadUnit e;
+    if (fast) {
+        int qx = p[0] - g.ox, qy = p[1] - g.oy, qz = p[2] - g.oz;
+        qx -= qx >= g.ex ? g.nx : 0;
+        qy -= qy >= g.ey ? g.ny : 0;
+        qz -= qz >= g.ez ? g.nz : 0;
+        const unsigned mx = axis_mask4(qx, g.ex), my = axis_mask4(qy, g.ey), mz = axis_mask4(qz, g.ez);
+        uint64_t mask = 0;
 #pragma unroll
-    for (int k0 = 0; k0 < kSpreadChunk; k0 += kSpreadBatch) {
-        if (k0 >= nq) break;
-        int cell[kSpreadBatch];
-        double val[kSpreadBatch];
+        for (int z = 0; z < 4; z++)
 #pragma unroll
-        for (int j = 0; j < kSpreadBatch; j++) {
-            const int k = k0 + j;
-            const T w = (wz[j] * wy[j]) * wx[j];
-            const int c = __builtin_amdgcn_readlane(cm, k);
-            const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, k) << 32) |
-                               (uint32_t)__builtin_amdgcn_readlane((int)mlo, k);
-            cell[j] = lane_select(k < nq ? m : 0, c + lofs, sink);
-            val[j] = (double)w;
-        }
-        if (k0 + kSpreadBatch < kSpreadChunk) {
+            for (int y = 0; y < 4; y++) {
+                const uint64_t row = ((mz >> z) & (my >> y) & 1u) ? mx : 0u;
+                mask |= row << (16 * z + 4 * y);
+            }
+        e.a = 8 * (d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx);  // bytes
+        e.b = 0;
+        e.mask = mask;
+    } else {
+        e.a = p[0];
+        e.b = p[1];
+        e.mask = (uint64_t)p[2];
+    }
+    units[u] = e;
+}
+
+// Wave wv adds the units u = wv, wv + 4, ... < 3 nq of a prepared chunk, lane
+// = stencil point (li, lj, lk), each lane into a distinct cell or, outside
+// the tile, into its own sink slot that is never read.
+template <typename T>
+__device__ __forceinline__ void spread_add(unsigned long long *acc, const T *w, const SpreadUnit *units, int nq,
+                                           int lane, int wv, const SpreadTile &g, bool fast, double scale) {
+    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
+    char *accb = (char *)acc;
+    const int sink = 8 * (3 * kTileCells + lane);
+    if (fast) {
+        const int lofs = 8 * (li + g.te0 * (lj + g.te1 * lk));  // bytes
+        // four units per round with all their LDS reads issued first; the
+        // units past 3 nq in the last round are read (inside the buffers)
+        // but add into the sinks
+        constexpr int U = 4;
+        const int nu = __builtin_amdgcn_readfirstlane(3 * nq);
+        for (int u0 = wv; u0 < nu; u0 += 4 * U) {
+            int cm[U];
+            uint64_t m[U];
+            T v[U];
 #pragma unroll
-            for (int j = 0; j < kSpreadBatch; j++) {
-                const SpreadRec<T> &B = Q[k0 + kSpreadBatch + j];
-                wz[j] = B.w[d][2][lk];
-                wy[j] = B.w[d][1][lj];
-                wx[j] = B.w[d][0][li];
+            for (int j = 0; j < U; j++) {
+                const int u = min(u0 + 4 * j, 3 * kSpreadChunk - 1);
+                cm[j] = __builtin_amdgcn_readfirstlane(units[u].a);
+                m[j] = uniform64(u0 + 4 * j < nu ? units[u].mask : 0ull);
+                const T *wu = w + 12 * u;
+                v[j] = (wu[8 + lk] * wu[4 + lj]) * wu[li];
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const int cell = lane_select(m[j], cm[j] + lofs, sink);
+                __hip_atomic_fetch_add((unsigned long long *)(accb + cell), fixed_term((double)v[j], scale),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        // in bead order: a cell met by several beads of the batch is added
-        // to in the order of the sorted list (LDS atomic adds from one wave
-        // are applied in program order; the lanes of one add meet distinct cells)
-#pragma unroll
-        for (int j = 0; j < kSpreadBatch; j++)
-            __hip_atomic_fetch_add((double *)(accb + cell[j]), val[j], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        for (int u = wv; u < 3 * nq; u += 4) {
+            const int d = u - 3 * (u / 3);
+            const SpreadUnit e = units[u];
+            int cx = e.a + li;
+            cx -= cx >= g.nx ? g.nx : 0;
+            int cy = e.b + lj;
+            cy -= cy >= g.ny ? g.ny : 0;
+            int cz = (int)e.mask + lk;
+            cz -= cz >= g.nz ? g.nz : 0;
+            cx -= g.ox;
+            cy -= g.oy;
+            cz -= g.oz;
+            const bool in =
+                (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
+            const T *wu = w + 12 * u;
+            const T v = (wu[8 + lk] * wu[4 + lj]) * wu[li];
+            const int cell = in ? 8 * (d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx) : sink;
+            __hip_atomic_fetch_add((unsigned long long *)(accb + cell), fixed_term((double)v, scale),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
 }
 
+// LDS of one wave-group (256 threads) of the spread: a chunk's staged
+// entries, factors and units, and the overflow window's bead list.
+template <typename T>
+struct SpreadGroup {
+    double ent[kSpreadChunk * 6];  // s[3], t_f[3] per bead
+    T w[3 * kSpreadChunk * 12];
+    SpreadUnit units[3 * kSpreadChunk];
+    int list[kSpreadChunk];
+    int nsel;
+    int pad[3];
+};
+struct SpreadShared {
+    unsigned long long acc[3 * kTileCells + 64];  // cell sums, then one sink slot per lane
+    int emax;
+    int pad[3];
+};
+
+// Tile t with cnt bucket insertions this step, by NG wave-groups of 256
+// threads: the scale from every impulse of the tile, then bucket chunk
+// ch = NG k + g and overflow window NG k + g by group g in round k, all
+// adding into one LDS accumulator; the tile is rounded to T and written to
+// the pool.  Every thread runs the same sequence of barriers.
+template <typename T, int NG>
+__device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uint32_t step, int par,
+                                            SpreadShared &sh, SpreadGroup<T> *grp) {
+    const int tid = threadIdx.x, lt = tid & 255, lane = tid & 63;
+    const int gi = __builtin_amdgcn_readfirstlane(tid >> 8);
+    const int wv = __builtin_amdgcn_readfirstlane(lt >> 6);
+    SpreadGroup<T> &G = grp[gi];
+    const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
+    const int te0 = R.te[0], te1 = R.te[1];
+    const int tcells = te0 * te1 * R.te[2];
+    // no stencil wraps back into the tile it starts in (see SpreadUnit)
+    const bool fast = nx >= te0 + 3 && ny >= te1 + 3 && nz >= R.te[2] + 3;
+    const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
+    const int ox = tx * te0, oy = ty * te1, oz = tz * R.te[2];
+    const SpreadTile tl{nx, ny, nz, ox, oy, oz, min(te0, nx - ox), min(te1, ny - oy), min(R.te[2], nz - oz), te0, te1};
+    const int nb = min(cnt, kBucketCap);
+    const int novf = cnt > kBucketCap ? min(R.ctr[par], kMaxTilesPerBead * R.M) : 0;
+    const SELM_GLOBAL f64x2 *src = gbl((const f64x2 *)R.bucket) + 3 * (long long)t * kBucketCap;
+    const int nch = (nb + kSpreadChunk - 1) / kSpreadChunk;
+
+    __syncthreads();  // a previous tile's use of LDS is over
+    if (tid == 0) sh.emax = 0;
+    for (int q = tid; q < 3 * kTileCells; q += 256 * NG) sh.acc[q] = 0ull;
+    // the scale: every impulse of the tile (bucket entry i's pieces 1 and 2,
+    // and the overflow pairs); the first round's entries load beside them
+    int e = 0;
+    {
+        const int nq0 = gi < nch ? min(kSpreadChunk, nb - gi * kSpreadChunk) : 0;
+        f64x2 st;
+        if (lt < 3 * nq0) st = src[3 * gi * kSpreadChunk + lt];
+        for (int i = tid; i < nb; i += 256 * NG) {
+            const f64x2 e1 = src[3 * i + 1], e2 = src[3 * i + 2];
+            e = max(e, max(exp_bits(e1.y), max(exp_bits(e2.x), exp_bits(e2.y))));
+        }
+        for (int o = tid; o < novf; o += 256 * NG)
+            if (R.ovf[2 * o] == t) {
+                const int bi = R.ovf[2 * o + 1];
+#pragma unroll
+                for (int d = 0; d < 3; d++) e = max(e, exp_bits(gbl(R.tf)[3 * bi + d]));
+            }
+        if (lt < 3 * nq0) ((f64x2 *)G.ent)[lt] = st;
+    }
+    // the wave's largest exponent by shuffles, then one LDS atomic per wave
+    // (an atomic from every lane becomes a per-lane scalar loop)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) e = max(e, __shfl_xor(e, m, 64));
+    __syncthreads();  // emax = 0 is seen by every thread
+    if (lane == 0) atomicMax(&sh.emax, e);
+
+    for (int k = 0; k * NG < nch; k++) {
+        const int ch = k * NG + gi;
+        const int nq = ch < nch ? min(kSpreadChunk, nb - ch * kSpreadChunk) : 0;
+        if (k > 0) {
+            __syncthreads();  // the previous round is consumed
+            if (lt < 3 * nq) ((f64x2 *)G.ent)[lt] = src[3 * ch * kSpreadChunk + lt];
+        }
+        __syncthreads();
+        if (lt < 3 * nq) spread_prepare<T>(G.ent, G.w, G.units, lt, R.f4, tl, fast);
+        __syncthreads();
+        if (nq > 0) spread_add<T>(sh.acc, G.w, G.units, nq, lane, wv, tl, fast, ldexp(1.0, spread_shift(sh.emax)));
+    }
+    // the overflow pairs of this tile, window by window
+    const int nwin = (novf + kSpreadChunk - 1) / kSpreadChunk;
+    for (int k = 0; k * NG < nwin; k++) {
+        const int win = k * NG + gi, o0 = win * kSpreadChunk;
+        __syncthreads();
+        if (lt == 0) G.nsel = 0;
+        __syncthreads();
+        if (win < nwin && lt < kSpreadChunk && o0 + lt < novf && R.ovf[2 * (o0 + lt)] == t)
+            G.list[atomicAdd(&G.nsel, 1)] = R.ovf[2 * (o0 + lt) + 1];
+        __syncthreads();
+        const int nq = __builtin_amdgcn_readfirstlane(G.nsel);
+        for (int i = lt; i < nq * 6; i += 256) {
+            const int kk = i / 6, q = i - 6 * kk, bi = G.list[kk];
+            G.ent[i] = q < 3 ? gbl(R.sg)[3 * bi + q] : gbl(R.tf)[3 * bi + q - 3];
+        }
+        __syncthreads();
+        if (lt < 3 * nq) spread_prepare<T>(G.ent, G.w, G.units, lt, R.f4, tl, fast);
+        __syncthreads();
+        if (nq > 0) spread_add<T>(sh.acc, G.w, G.units, nq, lane, wv, tl, fast, ldexp(1.0, spread_shift(sh.emax)));
+    }
+    __syncthreads();
+    const double inv = ldexp(1.0, -spread_shift(sh.emax));
+    SELM_GLOBAL T *pool = gbl((T *)R.pool) + (long long)t * 3 * tcells;
+    for (int q = tid; q < 3 * tcells; q += 256 * NG) {
+        const int d = q / tcells;
+        pool[q] = (T)((double)(long long)sh.acc[d * kTileCells + (q - d * tcells)] * inv);
+    }
+    if (tid == 0) {
+        if (cnt > kMaxTileBeads) atomicOr(R.status, kStatusTileOverflow);
+        R.bcnt[t] = 0;  // ready for the next step's insertions
+        R.tstamp[t] = (int)step;
+    }
+}
+
+// Tiles of at most kSplitAt bucket entries: one workgroup per tile.
 template <typename T>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps, int nrep,
                                                            const long long *__restrict__ start,
                                                            long long total) {
-    static_assert(3 * kTileCells * sizeof(double) >= kListCap * sizeof(int), "list shares the accumulators");
-    __shared__ int sorted[kListCap];
-    __shared__ double acc[3 * kTileCells + 64];  // the unsorted list first, then the cell sums (+ per-lane sinks)
-    __shared__ int nlist;
-    __shared__ __align__(16) SpreadRec<T> rec[2][kSpreadChunk];  // chunk c in rec[c & 1]
-    int *list = (int *)acc;
-
+    __shared__ SpreadShared sh;
+    __shared__ SpreadGroup<T> grp[1];
     const long long g = blockIdx.x;
     if (g >= total) return;
     const int r = find_rep(start, nrep, g);
     const RepDev &R = reps[r];
     const int t = (int)(g - start[r]);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t step = (uint32_t)(R.ts[0] + 1);
     const int par = step & 1;
-    if (t == 0 && tid == 0) R.ctr[par ^ 1] = 0;  // next step's overflow count (last used a step ago)
+    if (t == 0 && threadIdx.x == 0) {
+        // the next step's counters (last used a step ago)
+        R.ctr[par ^ 1] = 0;
+        R.ctr[kCtrSplits + (par ^ 1)] = 0;
+    }
     const int cnt = R.bcnt[t];
-    if (cnt == 0) return;  // no bead touches this tile: its stamp stays stale
+    // untouched tiles keep a stale stamp; split tiles are k_spread_split's
+    if (cnt == 0 || cnt > kSplitAt) return;
     // Issue priority over the thermal forcing that runs beside the spread on
-    // the forked stream: the step waits for the spread's fullest tiles, not
-    // for the forcing, which has the first FFT pass to finish in.
+    // the forked stream: the step waits for the spread, not for the forcing,
+    // which has the first FFT pass to finish in.
     __builtin_amdgcn_s_setprio(3);
-    const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
-    const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
-    const int ox = tx * R.te[0], oy = ty * R.te[1], oz = tz * R.te[2];
-    const int ex = min(R.te[0], nx - ox), ey = min(R.te[1], ny - oy), ez = min(R.te[2], nz - oz);
+    spread_tile<T, 1>(R, t, cnt, step, par, sh, grp);
+}
 
-    const int nb = min(cnt, kBucketCap);
-    if (tid == 0) nlist = nb;
-    for (int i = tid; i < nb; i += kSpreadThreads) list[i] = gbl(R.bucket)[t * kBucketCap + i];
-    __syncthreads();
-    if (cnt > kBucketCap) {
-        const int novf = R.ctr[par];
-        for (int o = tid; o < novf; o += kSpreadThreads)
-            if (R.ovf[2 * o] == t) {
-                const int p = atomicAdd(&nlist, 1);
-                if (p < kListCap) list[p] = R.ovf[2 * o + 1];
-            }
-        __syncthreads();
+// Tiles of more than kSplitAt bucket entries, from the split list the bead
+// update filled: four wave-groups add four chunks at once.  Workgroup j of a replica takes
+// split tiles j, j + G, ... (G = its workgroups).  The LDS is dynamic
+// (SpreadShared, then 4 SpreadGroup<T>; more than the 64 KB of static LDS).
+constexpr int kSplitGroups = 4;
+template <typename T>
+__global__ __launch_bounds__(256 * kSplitGroups) void k_spread_split(const RepDev *__restrict__ reps, int nrep,
+                                                                     const long long *__restrict__ start,
+                                                                     long long total) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    SpreadShared &sh = *(SpreadShared *)smem;
+    SpreadGroup<T> *grp = (SpreadGroup<T> *)(smem + sizeof(SpreadShared));
+    const long long g = blockIdx.x;
+    if (g >= total) return;
+    const int r = find_rep(start, nrep, g);
+    const RepDev &R = reps[r];
+    const int j = (int)(g - start[r]), G = (int)(start[r + 1] - start[r]);
+    const uint32_t step = (uint32_t)(R.ts[0] + 1);
+    const int par = step & 1;
+    const int nsplit = R.ctr[kCtrSplits + par];
+    if (j >= nsplit) return;
+    __builtin_amdgcn_s_setprio(3);
+    for (int k = j; k < nsplit; k += G) {
+        const int t = gbl(R.slist)[k];
+        spread_tile<T, kSplitGroups>(R, t, R.bcnt[t], step, par, sh, grp);
     }
-    const int n = min(nlist, kListCap);
-    if (tid == 0) {
-        if (nlist > kListCap) atomicOr(&R.ctr[4], kStatusListOverflow);
-        R.bcnt[t] = 0;  // ready for the next step's insertions
-    }
-    // order the tile's beads by index (indices are unique within a tile)
-    for (int e = tid; e < n; e += kSpreadThreads) {
-        const int bi = list[e];
-        int rank = 0;
-        for (int q = 0; q < n; q++) rank += (list[q] < bi) ? 1 : 0;
-        sorted[rank] = bi;
-    }
-    __syncthreads();  // sorted is complete and the list is read: its space becomes the accumulators
-    const SpreadRec<T> *brec = (const SpreadRec<T> *)R.brec;
-    spread_stage<T, kSpreadThreads>(rec[0], brec, sorted, 0, min(kSpreadChunk, n), tid);
-    for (int c = tid; c < 3 * kTileCells; c += kSpreadThreads) acc[c] = 0.0;
-    __syncthreads();
-    spread_fold_force<T, kSpreadThreads>(rec[0], min(kSpreadChunk, n), tid);
-    __syncthreads();
+}
 
-    // Waves 0..2 add chunk c (wave d: component d, lane: stencil point) while
-    // wave 3 stages chunk c+1 into the other buffer.
-    const int te0 = R.te[0], te1 = R.te[1];
-    // no stencil wraps back into the tile it starts in (see spread_add_chunk)
-    const bool fast = nx >= te0 + 3 && ny >= te1 + 3 && nz >= R.te[2] + 3;
-    for (int c = 0, base = 0; base < n; c++, base += kSpreadChunk) {
-        const int nq = min(kSpreadChunk, n - base);
-        if (wv == 3) {
-            const int nb2 = base + kSpreadChunk;
-            if (nb2 < n) {
-                const int nq2 = min(kSpreadChunk, n - nb2);
-                spread_stage<T, 64>(rec[(c + 1) & 1], brec, sorted, nb2, nq2, lane);
-                // the fold reads what other lanes of this wave just stored:
-                // LDS operations of one wave complete in order
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                spread_fold_force<T, 64>(rec[(c + 1) & 1], nq2, lane);
-            }
-        } else {
-            const SpreadTile tl{nx, ny, nz, ox, oy, oz, ex, ey, ez, te0, te1};
-            if (fast) spread_add_chunk_fast<T>(acc, rec[c & 1], nq, wv, lane, tl);
-            else spread_add_chunk<T>(acc, rec[c & 1], nq, wv, lane, tl);
-        }
-        __syncthreads();  // chunk c consumed, chunk c+1 staged
-    }
-    const int tcells = te0 * te1 * R.te[2];
-    SELM_GLOBAL T *pool = gbl((T *)R.pool) + (long long)t * 3 * tcells;
-    for (int c = tid; c < 3 * tcells; c += kSpreadThreads) {
-        const int d = c / tcells, cell = d * kTileCells + (c - d * tcells);
-        pool[c] = (T)acc[cell];
-    }
-    if (tid == 0) R.tstamp[t] = (int)step;
+size_t spread_split_lds(bool fp64) {
+    return sizeof(SpreadShared) +
+           kSplitGroups * (fp64 ? sizeof(SpreadGroup<double>) : sizeof(SpreadGroup<float>));
+}
+
+hipError_t spread_split_prepare() {
+    hipError_t e = hipFuncSetAttribute((const void *)k_spread_split<float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)spread_split_lds(false));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_spread_split<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)spread_split_lds(true));
+    return e;
 }
 
 // u* over the whole grid into fsp (runtime-length FFT path only)
@@ -1010,6 +1085,21 @@ static inline unsigned nblk(long long n, int per) { return (unsigned)((n + per -
 template <typename T>
 static void launch_fft_generic(const StepLaunch &L, int stage, hipStream_t st);
 
+// the one-chunk tiles of the spread, and u* on the runtime-length FFT path
+template <typename T>
+static void launch_spread_small_t(const StepLaunch &L, hipStream_t st) {
+    hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads), 0, st, L.reps, L.nrep,
+                       L.st_tile, L.tot_tile);
+    if (!L.fast)
+        hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, L.reps, L.nrep, L.st_pts,
+                           L.tot_pts);
+}
+
+static void launch_spread_small(const StepLaunch &L, hipStream_t st) {
+    if (L.fp64) launch_spread_small_t<double>(L, st);
+    else launch_spread_small_t<float>(L, st);
+}
+
 template <typename T>
 static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
     const RepDev *reps = L.reps;
@@ -1024,11 +1114,8 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
                                    st, reps, nrep, L.st_bead, L.tot_bead);
             break;
         case kStageSpread:
-            hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads), 0, st, reps, nrep,
-                               L.st_tile, L.tot_tile);
-            if (!L.fast)
-                hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep, L.st_pts,
-                                   L.tot_pts);
+            launch_spread_split(L, st);
+            launch_spread_small_t<T>(L, st);
             break;
         case kStageFftX:
         case kStageFftY:
@@ -1072,6 +1159,16 @@ static void launch_fft_generic(const StepLaunch &L, int stage, hipStream_t st) {
     }
 }
 
+void launch_spread_split(const StepLaunch &L, hipStream_t st) {
+    if (L.tot_split <= 0) return;
+    if (L.fp64)
+        hipLaunchKernelGGL(k_spread_split<double>, dim3((unsigned)L.tot_split), dim3(256 * kSplitGroups),
+                           spread_split_lds(true), st, L.reps, L.nrep, L.st_split, L.tot_split);
+    else
+        hipLaunchKernelGGL(k_spread_split<float>, dim3((unsigned)L.tot_split), dim3(256 * kSplitGroups),
+                           spread_split_lds(false), st, L.reps, L.nrep, L.st_split, L.tot_split);
+}
+
 void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
     if (L.fp64) launch_stage_t<double>(L, stage, st);
     else launch_stage_t<float>(L, stage, st);
@@ -1088,16 +1185,23 @@ void launch_step(const StepLaunch &L, hipStream_t st) {
     const bool fork = L.tot_modes > 0;
     launch_stage(L, kStageBead, st);
     if (fork) {
+        // the forked branch: the split tiles of the spread (joined before the
+        // X pass, which reads the pool), then the thermal forcing and the
+        // conservative forces (joined before the Z pass)
         (void)hipEventRecord(L.ev_fork, st);
         (void)hipStreamWaitEvent(L.st2, L.ev_fork, 0);
+        launch_spread_split(L, L.st2);
+        (void)hipEventRecord(L.ev_split, L.st2);
         launch_noise(L, L.st2);
         launch_stage(L, kStageBonds, L.st2);
         (void)hipEventRecord(L.ev_join, L.st2);
     }
     for (int s = kStageSpread; s < kNumStages; s++) {
         if (fork && s == kStageBonds) continue;
+        if (s == kStageFftX && fork) (void)hipStreamWaitEvent(st, L.ev_split, 0);
         if (s == kStageFftZ && fork) (void)hipStreamWaitEvent(st, L.ev_join, 0);
-        launch_stage(L, s, st);
+        if (s == kStageSpread && fork) launch_spread_small(L, st);
+        else launch_stage(L, s, st);
     }
 }
 
@@ -1120,6 +1224,5 @@ void launch_fill_f32(float *p, long long n, float v, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
 }
 
-size_t spread_rec_bytes(bool fp64) { return fp64 ? sizeof(SpreadRec<double>) : sizeof(SpreadRec<float>); }
 
 }  // namespace selm

@@ -22,6 +22,12 @@ SELM_ERR_STATE = 3
 SELM_ERR_NOMEM = 4
 SELM_ERR_UNSUPPORTED = 5
 SELM_ERR_POOL_FULL = 6
+SELM_ERR_DEVICE = 7
+
+# device status flags (selm_status / selm_pool_status)
+STATUS_TILE_OVERFLOW = 1
+STATUS_NONFINITE = 2
+STATUS_DISPLACEMENT = 4
 
 PREC_FP64 = 0
 PREC_FP32_BULK = 1
@@ -38,7 +44,7 @@ EXPORTS = [
     "selm_pool_step", "selm_pool_sync", "selm_pool_get_beads", "selm_pool_get_field",
     "selm_pool_timestep", "selm_pool_retire", "selm_pool_stats", "selm_pool_stream",
     "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes", "selm_autotune",
-    "selm_pool_autotune",
+    "selm_pool_autotune", "selm_status", "selm_pool_status", "selm_check_beads",
 ]
 
 
@@ -113,6 +119,9 @@ def load(path=None):
                                   C.c_int, ip, dp, ip]
     L.selm_pool_step.argtypes = [vp, C.c_int]
     L.selm_pool_sync.argtypes = [vp]
+    L.selm_status.argtypes = [vp, ip, C.c_int]
+    L.selm_check_beads.argtypes = [C.c_int, dp, dp, dp, ip]
+    L.selm_pool_status.argtypes = [vp, C.c_int, ip, C.c_int]
     L.selm_pool_get_beads.argtypes = [vp, C.c_int, dp, dp, dp, dp]
     L.selm_pool_get_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.selm_pool_timestep.argtypes = [vp, C.c_int]
@@ -154,6 +163,17 @@ def make_params(dims, lo, hi, rho=1.0, mu=1.0, kB=1.0, T=1.0, dt=0.01, seed=1234
     p.precision = int(precision)
     p.f4_quirk = int(bool(f4_quirk))
     return p
+
+
+def check_beads(x, v, m):
+    """selm_check_beads: -1 when the bead inputs are valid, else the first
+    offending bead (runs on the host, no device needed)."""
+    x = f64(x, (-1, 3))
+    v = None if v is None else f64(v, (-1, 3))
+    m = f64(m, (-1,))
+    bad = C.c_int(-1)
+    load().selm_check_beads(x.shape[0], _d(x), _d(v), _d(m), C.byref(bad))
+    return bad.value
 
 
 def footprint(params, M, nb):
@@ -239,7 +259,15 @@ class System:
         self._chk(self.L.selm_step(self.h, int(n), stream), "step")
 
     def sync(self):
+        """Wait for the system's work; raises SelmError (code SELM_ERR_DEVICE)
+        when a device status flag is raised."""
         self._chk(self.L.selm_sync(self.h), "sync")
+
+    def status(self, clear=False):
+        """Device status flags (STATUS_*), after waiting for the work."""
+        f = C.c_int(0)
+        self._chk(self.L.selm_status(self.h, C.byref(f), int(bool(clear))), "status")
+        return f.value
 
     def beads(self):
         M = self.M
@@ -349,7 +377,15 @@ class Pool:
         self._chk(self.L.selm_pool_step(self.h, int(n)), "pool_step")
 
     def sync(self):
+        """Wait for the pool's work; raises SelmError (SELM_ERR_DEVICE) when an
+        admitted system has a status flag raised."""
         self._chk(self.L.selm_pool_sync(self.h), "pool_sync")
+
+    def status(self, slot, clear=False):
+        """Device status flags (STATUS_*) of one admitted system."""
+        f = C.c_int(0)
+        self._chk(self.L.selm_pool_status(self.h, int(slot), C.byref(f), int(bool(clear))), "pool_status")
+        return f.value
 
     def beads(self, slot):
         M, _ = self.meta[slot]

@@ -10,7 +10,12 @@ number of steps.  The loop works on requests as follows:
 - it advances every admitted system by one step with one batched launch per
   stage;
 - a system that has run its steps has its final bead state read back and is
-  retired, freeing its slot and memory for the next arrival.
+  retired, freeing its slot and memory for the next arrival;
+- every `status_every` ticks, and before a result is read, the pool's device
+  status flags are checked, and a system whose flags are raised (a bead
+  position or velocity gone non-finite, a displacement beyond half the box,
+  a spread tile out of range) is evicted: retired with its flags recorded,
+  while the other systems carry on unchanged.
 
 Requests that arrive while others are on the GPU therefore join the running
 batch at the next tick.  Each system keeps its own step counter, noise
@@ -47,6 +52,7 @@ class Request:
         self.admitted_at = None
         self.done_at = None
         self.result = None  # (x, v) after `steps` steps
+        self.failed = 0     # device status flags that evicted the run (0: none)
 
 
 class _Events:
@@ -89,12 +95,14 @@ class _Events:
 class Server:
     """Continuous batching of SELM runs over one selm_pool."""
 
-    def __init__(self, hbm_budget, max_slots, precision=selm.PREC_FP32_BULK, time_steps=True):
+    def __init__(self, hbm_budget, max_slots, precision=selm.PREC_FP32_BULK, time_steps=True, status_every=16):
         self.pool = selm.Pool(hbm_budget, max_slots, precision=precision)
         self.precision = precision
         self.queue = deque()
         self.running = {}  # slot -> (request, steps done)
         self.finished = []
+        self.evicted = []
+        self.status_every = status_every
         self.step_ms = []
         self.bead_steps = 0
         self.ticks = 0
@@ -126,6 +134,30 @@ class Server:
             self.ev.release(a)
             self.ev.release(b)
 
+    def _evict(self, sl, flags, now):
+        r = self.running.pop(sl)[0]
+        r.failed = flags
+        r.done_at = now
+        self.pool.retire(sl)
+        self.evicted.append(r)
+
+    def check_status(self, now=0.0):
+        """Wait for the pool and evict every running system whose device
+        status flags are raised; returns the evicted requests."""
+        try:
+            self.pool.sync()
+            return []
+        except selm.SelmError as e:
+            if e.code != selm.SELM_ERR_DEVICE:
+                raise
+        out = []
+        for sl in list(self.running):
+            f = self.pool.status(sl)
+            if f:
+                out.append(self.running[sl][0])
+                self._evict(sl, f, now)
+        return out
+
     def tick(self, now=0.0):
         """Admit what fits, advance every running system by one step, and
         retire the systems that have run their steps.  Returns the number of
@@ -148,8 +180,16 @@ class Server:
             ent = self.running[sl]
             ent[1] += 1
             self.bead_steps += ent[0].w.M
+        if self.status_every and self.ticks % self.status_every == 0:
+            self.check_status(now)
+        for sl in list(self.running):
+            ent = self.running[sl]
             if ent[1] == ent[0].steps:
                 r = ent[0]
+                f = self.pool.status(sl)
+                if f:
+                    self._evict(sl, f, now)
+                    continue
                 x, v, _, _ = self.pool.beads(sl)
                 r.result = (x, v)
                 r.done_at = now
@@ -237,7 +277,8 @@ def run(duration=30.0, rate=20.0, budget_gb=16.0, slots=64, seed=5, precision=se
     srv.close()
     lat = srv.latency()
     waits = [r.admitted_at - r.arrival for r in srv.finished]
-    return {"requests": len(reqs), "finished": len(srv.finished), "wall_s": round(wall, 3),
+    return {"requests": len(reqs), "finished": len(srv.finished), "evicted": len(srv.evicted),
+            "wall_s": round(wall, 3),
             "arrival_rate_per_s": rate, "hbm_budget_gb": budget_gb, "slots": slots,
             "peak_concurrent": peak, "bead_steps_per_s": round(srv.bead_steps / wall, 1),
             "batched_steps": srv.ticks, "step_latency": lat,

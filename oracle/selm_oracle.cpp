@@ -39,6 +39,7 @@
  */
 #include "selm_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -573,6 +574,11 @@ int oracle_init(void *op) {
             double &xi = o.x[3 * i + d];
             if (xi < o.P.lo[d]) xi += L;
             if (xi >= o.P.hi[d]) xi -= L;
+            if (!(xi >= o.P.lo[d] && xi < o.P.hi[d])) {  /* more than one box length outside */
+                xi = o.P.lo[d] + ((xi - o.P.lo[d]) - L * std::floor((xi - o.P.lo[d]) / L));
+                if (xi >= o.P.hi[d]) xi -= L;
+            }
+            xi = std::max(xi, o.P.lo[d]);
         }
     bond_forces(o, o.x.data(), o.fc.data());
     std::fill(o.u[0].begin(), o.u[0].end(), 0.0);

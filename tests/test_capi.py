@@ -47,3 +47,28 @@ def test_unsupported_edges_rejected(edge):
     with pytest.raises(selm.SelmError) as e:
         selm.footprint(p, 1, 0)
     assert e.value.code == selm.SELM_ERR_UNSUPPORTED
+
+
+def test_check_beads_rejects_non_finite_inputs():
+    """selm_check_beads (the check selm_set_beads and selm_pool_admit apply)
+    refuses NaN/Inf positions or velocities and masses that are not finite
+    and > 0, naming the first offending bead; it runs without a device."""
+    import numpy as np
+    from monomers2026_amd import selm
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0, 8, (10, 3))
+    v = rng.standard_normal((10, 3))
+    m = np.ones(10)
+    assert selm.check_beads(x, v, m) == -1
+    assert selm.check_beads(x, None, m) == -1
+    for bad, (arr, val) in enumerate([("x", np.nan), ("v", np.inf), ("m", 0.0), ("x", -np.inf), ("m", np.nan),
+                                      ("m", -1.0), ("v", np.nan)]):
+        xx, vv, mm = x.copy(), v.copy(), m.copy()
+        i = 1 + bad
+        if arr == "x":
+            xx[i, bad % 3] = val
+        elif arr == "v":
+            vv[i, bad % 3] = val
+        else:
+            mm[i] = val
+        assert selm.check_beads(xx, vv, mm) == i, (arr, val)

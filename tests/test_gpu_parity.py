@@ -276,3 +276,54 @@ def test_thermal_forcing_grid_stride(gpu_lib):
         print("grid-stride forcing, precision %d, fft passes %d" % (prec, s.fft_passes()), flush=True)
         _compare(s, o, tol, "grid-stride forcing %s precision=%d" % (dims, prec), box=float(dims[0]))
         s.close()
+
+
+@pytest.mark.parametrize("noise", [False, True])
+def test_chain64_32_textbook_kernel_fp64(gpu_lib, noise):
+    """f4_quirk = 0 (the textbook Peskin kernel: factor 0 at r >= 2) against
+    the oracle with the same switch.  Bead 0 starts exactly at the box
+    centre, where the reference's kernel keeps the factor 1.0; with random
+    initial velocities the two modes differ from the first step, so this
+    also shows the switch reaches the device."""
+    from monomers2026_amd import selm, workloads
+    w = workloads.chain64_32()
+    w.v = np.random.default_rng(2).standard_normal(w.x.shape)
+    sysg = w.build_system(selm.PREC_FP64, noise, seed=7, f4_quirk=False)
+    o = _oracle(w, noise, 7, f4=False)
+    oq = _oracle(w, noise, 7, f4=True)
+    for n in (1, 19):
+        sysg.step(n)
+        o.step(n)
+        oq.step(n)
+        _compare(sysg, o, FP64_TOL, "chain64_32 f4=0 noise=%d step=%d" % (noise, o.timestep))
+    assert _rel(o.beads()[1], oq.beads()[1]) > 1e-6  # the quirk changes the result
+    o.close()
+    oq.close()
+    sysg.close()
+
+
+@pytest.mark.parametrize("noise", [False, True])
+def test_mean_mode_is_bead_impulse(gpu_lib, noise):
+    """Momentum bookkeeping (cuda/SELM_Kernel.cu:339-354, the k=0 removal of
+    cuda/SELM_Eulerian_Period.cu:121-125): from u = 0, rho dx^3 sum(u*) at
+    step 1 equals the impulse the beads hand the fluid, -sum_b t_f (drag and
+    thermal kick), component by component, to 1e-12 of sum_b |t_f|.  Bead
+    velocities start random and rho != 1, so neither side is trivially
+    zero and a missing density factor fails."""
+    from monomers2026_amd import selm, workloads
+    w = workloads.chains1000_64()
+    w.rho = 1.7
+    w.v = np.random.default_rng(3).standard_normal(w.x.shape)
+    s = w.build_system(selm.PREC_FP64, noise, seed=13)
+    fc0 = s.beads()[2]
+    s.step(1)
+    ft = s.beads()[3]
+    tf = (ft - fc0) * w.dt  # f_total = (t_f + dt f_cons) / dt
+    dx = 1.0
+    got = w.rho * dx ** 3 * s.mean_mode()
+    want = -tf.sum(axis=0)
+    scale = np.abs(tf).sum(axis=0)
+    err = np.abs(got - want) / scale
+    print("mean mode noise=%d: rho dx^3 sum u* %s, -sum t_f %s, rel %s" % (noise, got, want, err), flush=True)
+    assert np.all(err <= 1e-12), err
+    s.close()
