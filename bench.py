@@ -67,11 +67,12 @@ def stage_bytes(w, s, passes=5):
     N = w.edge ** 3
     Nc = (w.edge // 2 + 1) * w.edge * w.edge
     M = w.M
-    rec = 208 if s == 4 else 368  # spread record written per bead
     ypass = 0 if passes == 3 else 2 * 3 * Nc * 2 * s
     return {
         "thermal_noise": 3 * Nc * 2 * s,
-        "bead_update": M * (192 * s + 24 + 152 + rec),
+        # stencil values, x read, bead state (x, v, f, m, t_f, s, f_total),
+        # one 48-byte bucket entry per (tile, bead) pair
+        "bead_update": M * (192 * s + 24 + 152) + 48 * tile_pairs(w),
         "spread": M * (24 + 3 * s) + 192 * M * s,
         "ustar_fft_x": 3 * N * s + 3 * Nc * 2 * s,
         "fft_y": ypass,
@@ -125,6 +126,24 @@ CLASSES = {
 }
 
 
+def tile_pairs(w):
+    """(tile, bead) pairs of the spread's buckets at the starting positions:
+    per axis the 8-cell tiles met by cells floor(s - 1/2) - 1 .. floor(s) + 2
+    (the union of the three staggered stencils)."""
+    L = w.edge
+    te = 8
+    xs = np.asarray(w.x, np.float64)
+    lo = (np.floor(xs - 0.5) - 1).astype(np.int64)
+    hi = np.floor(xs).astype(np.int64) + 2
+    n = np.ones(xs.shape[0], np.int64)
+    for a in range(3):
+        cells = lo[:, a, None] + np.arange(6)[None, :]
+        ok = cells <= hi[:, a, None]
+        tiles = np.where(ok, (cells % L) // te, -1)
+        n *= np.array([np.unique(r[r >= 0]).size for r in tiles])
+    return int(n.sum())
+
+
 def unique_stencil_bytes(w, s):
     """Compulsory stencil bytes: the distinct (component, cell) values that the
     4x4x4 stencils of all beads touch at the starting positions (box [0, L),
@@ -144,9 +163,15 @@ def unique_stencil_bytes(w, s):
     return n * s
 
 
+GRAPH_WARMUP = 5  # steps that capture and instantiate both step graphs (1 step and 4 steps)
+
+
 def time_steps(sysg, ev, steps, warmup, runs=1):
-    """Median over `runs` timed regions of `steps` graph replays each."""
-    sysg.step(warmup)
+    """Median over `runs` (odd) timed regions of `steps` graph replays each.
+    The warm-up is at least GRAPH_WARMUP steps, so both graphs exist before
+    the first timed region."""
+    assert runs % 2 == 1
+    sysg.step(max(warmup, GRAPH_WARMUP))
     sysg.sync()
     st = sysg.stream
     res = []
@@ -160,6 +185,47 @@ def time_steps(sysg, ev, steps, warmup, runs=1):
         res.append((ms, time.perf_counter() - t0))
     res.sort()
     return res[len(res) // 2]
+
+
+def displacement_error(w, steps=1000, seed=2026):
+    """FP32-bulk against the FP64 GPU path (equal to the oracle to 1e-12) on
+    the headline workload, outside the timed regions: max-norm bead
+    displacement error from x^0 (minimum image), relative to the largest
+    FP64 displacement, after 100 and `steps` steps."""
+    from monomers2026_amd import selm
+    s32 = w.build_system(selm.PREC_FP32_BULK, True, seed=seed)
+    s64 = w.build_system(selm.PREC_FP64, True, seed=seed)
+    x0 = s64.beads()[0]
+    L = float(w.edge)
+    out = {"steps": [], "rel_err": [], "max_displacement": []}
+    done = 0
+    for target in (100, steps):
+        s32.step(target - done)
+        s64.step(target - done)
+        done = target
+        d32 = s32.beads()[0] - x0
+        d64 = s64.beads()[0] - x0
+        d32 -= L * np.round(d32 / L)
+        d64 -= L * np.round(d64 / L)
+        out["steps"].append(target)
+        out["rel_err"].append(float(np.abs(d32 - d64).max() / np.abs(d64).max()))
+        out["max_displacement"].append(float(np.abs(d64).max()))
+    out["status"] = {"fp32": s32.status(), "fp64": s64.status()}
+    s32.close()
+    s64.close()
+    return out
+
+
+def served_leg(duration=15.0, rate=20.0):
+    """A short fixed-seed run of the served ensemble (Poisson arrivals, edges
+    32-128, 100-5000 beads in chains of 50, 200-2000 steps each, FP32 bulk,
+    continuous batching over a 16 GiB pool of 64 slots); the serving
+    configuration asks for 10 minutes (python -m monomers2026_amd.serve
+    --duration 600)."""
+    from monomers2026_amd import serve
+    rep = serve.run(duration=duration, rate=rate, budget_gb=16.0, slots=64, seed=5, progress_s=0)
+    rep["note"] = "%.0f s of arrivals, then drained; the 10-minute run is the serving configuration" % duration
+    return rep
 
 
 def cpu_model():
@@ -270,7 +336,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--runs", type=int, default=5, help="timed regions; the median is reported")
+    ap.add_argument("--runs", type=int, default=5, help="timed regions (odd); the median is reported")
+    ap.add_argument("--no-serve", action="store_true", help="skip the short served-ensemble leg")
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=50)
@@ -298,7 +366,8 @@ def main():
     # (timed on the system itself, whose state it restores)
     s32 = w.build_system(selm.PREC_FP32_BULK, stochastic=True, seed=2026)
     tune32 = s32.autotune(20)
-    ms32, wall32 = time_steps(s32, ev, args.steps, args.warmup, 1 if args.fp32_only else args.runs)
+    runs = max(1, args.runs | 1)
+    ms32, wall32 = time_steps(s32, ev, args.steps, args.warmup, 1 if args.fp32_only else runs)
     if args.fp32_only:
         s32.close()
         print(json.dumps({"fp32_only": True, "steps": args.steps, "ms_per_step": round(ms32 / args.steps, 5)}),
@@ -311,7 +380,7 @@ def main():
     # FP64 path on the same workload
     s64 = w.build_system(selm.PREC_FP64, stochastic=True, seed=2026)
     steps64 = max(1, args.steps // 2)
-    ms64, wall64 = time_steps(s64, ev, steps64, max(1, args.warmup // 2), max(1, args.runs // 2))
+    ms64, wall64 = time_steps(s64, ev, steps64, args.warmup, max(3, (runs // 2) | 1))
     passes64 = s64.fft_passes()
     stages64 = s64.stage_times(args.stage_steps)
     s64.close()
@@ -356,7 +425,8 @@ def main():
         "data": "synthetic: seeded random-walk chains, random-init fluid at rest",
         "config": {"workload": "dilute20k_128", "grid": [128, 128, 128], "beads": w.M, "chains": 200,
                    "beads_per_chain": 100, "external_force_beads": 1000, "noise": True, "dt": w.dt},
-        "timing": "median of %d timed regions of %d steps" % (args.runs, args.steps),
+        "timing": "median of %d timed regions of %d steps (FP64: %d of %d), after >= %d warm-up steps"
+                  % (runs, args.steps, max(3, (runs // 2) | 1), steps64, GRAPH_WARMUP),
         "grid_point_steps_per_s": round(N * args.steps / (ms32 * 1e-3), 1),
         "fp64": {"value": round(bead_steps64, 1), "ms_per_step": round(per64, 5), "steps": steps64},
         "fp32_over_fp64": round(bead_steps32 / bead_steps64, 3),
@@ -373,10 +443,18 @@ def main():
         "roofline": roof,
         "unique_stencil_bytes_fp32": unique_stencil_bytes(w, 4),
     }
+    out["fp32_bulk_displacement_error"] = displacement_error(w)
+    print("fp32-bulk displacement error vs fp64: %s" % out["fp32_bulk_displacement_error"]["rel_err"],
+          file=sys.stderr, flush=True)
     if not args.no_ensemble:
         out["ensemble256"] = ensemble_leg(ev)
         print("ensemble: p50 %.4f ms p99 %.4f ms" % (out["ensemble256"]["p50_ms"], out["ensemble256"]["p99_ms"]),
               file=sys.stderr, flush=True)
+    if not args.no_serve:
+        out["served"] = served_leg(args.serve_seconds)
+        lat = out["served"]["step_latency"]
+        print("served: p50 %.4f ms p99 %.4f ms, %.3g bead-steps/s" % (lat["p50_ms"], lat["p99_ms"],
+              out["served"]["bead_steps_per_s"]), file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

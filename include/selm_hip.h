@@ -188,8 +188,8 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
  * ctx stream; ms[s] receives the summed milliseconds of stage s
  * (0 thermal forcing, 1 bead update + binning, 2 spread, 3 viscous u* + FFT x,
  *  4 FFT y, 5 FFT z + projection, 6 inverse FFT y, 7 inverse FFT x,
- *  8 bond forces).  In selm_step the thermal forcing runs concurrently with
- *  stages 1-4; here every stage runs alone. */
+ *  8 bond forces).  In selm_step the thermal forcing and the bond forces run
+ *  on a forked stream beside stages 2-4; here every stage runs alone. */
 int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
 
 /* Autotuner of the FFT pass shapes (no reference counterpart: the

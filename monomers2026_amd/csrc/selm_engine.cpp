@@ -46,7 +46,7 @@ struct Geometry {
     long long N, Nc;
     RadixPlan plan[3];
     int te[3], tn[3], ntiles;
-    int gsplit;  // workgroups of the split-tile spread
+    int gsplit;  // spread workgroups that take the split tiles
     int bx, by, bz;
 };
 
@@ -100,7 +100,7 @@ static int make_geometry(const selm_params &p, bool fp64, Geometry &G, std::stri
     G.N = (long long)G.n[0] * G.n[1] * G.n[2];
     G.Nc = (long long)G.xc * G.n[1] * G.n[2];
     G.ntiles = G.tn[0] * G.tn[1] * G.tn[2];
-    G.gsplit = std::max(1, G.ntiles / 16);
+    G.gsplit = std::max(1, G.ntiles / 8);
     const size_t csz = fp64 ? 16 : 8;
     G.bx = fp64 ? 16 : 32;
     G.by = fp64 ? 8 : 16;
@@ -700,7 +700,7 @@ struct Batch {
     int zq_cols = 17;               // plane path: Z-pass columns per workgroup (17 or 9)
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_split = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<Replica *> reps;
     RepDev *d_reps = nullptr;
     int cap_reps = 0;
@@ -799,11 +799,12 @@ struct Batch {
             const Geometry &G = R.G;
             h[r] = R.D;
             h[r].M = R.M;
+            h[r].split = R.M >= kSplitMinBeads ? 1 : 0;
             sb[r + 1] = sb[r] + R.M;
             stl[r + 1] = stl[r] + G.ntiles;
             sen[r + 1] = sen[r] + G.N;
             smo[r + 1] = smo[r] + G.Nc;
-            ssp[r + 1] = ssp[r] + G.gsplit;
+            ssp[r + 1] = ssp[r] + (R.M >= kSplitMinBeads ? G.gsplit : 0);
             any_noise = any_noise || R.D.noise;
             if (plane) {
                 long long ixy, iz;
@@ -866,7 +867,6 @@ struct Batch {
         L.st2 = stream2;
         L.ev_fork = ev_fork;
         L.ev_join = ev_join;
-        L.ev_split = ev_split;
         L.lds_fx = lfx;
         L.lds_fy = lfy;
         L.lds_fz = lfz;
@@ -890,8 +890,7 @@ struct Batch {
         HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&ev_split, hipEventDisableTiming));
-        HIPCHK(spread_split_prepare());
+
         return SELM_OK;
     }
 
@@ -944,10 +943,9 @@ struct Batch {
         if (stream2) (void)hipStreamDestroy(stream2);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
-        if (ev_split) (void)hipEventDestroy(ev_split);
         stream = nullptr;
         stream2 = nullptr;
-        ev_fork = ev_join = ev_split = nullptr;
+        ev_fork = ev_join = nullptr;
     }
 };
 

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
             e[2] = f64x2{t1, t2};
             // a tile of more than kSplitAt entries goes to the split list
             // (once per step: the count passes kSplitAt once)
-            if (slot == kSplitAt) gbl(R.slist)[atomicAdd(&R.ctr[kCtrSplits + (step & 1)], 1)] = t;
+            if (slot == kSplitAt && R.split) gbl(R.slist)[atomicAdd(&R.ctr[kCtrSplits + (step & 1)], 1)] = t;
         } else {
             const int o = atomicAdd(&R.ctr[step & 1], 1);
             gbl(R.ovf)[2 * o] = t;
@@ -504,18 +504,42 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     }
 }
 
-// Tiles of at most kSplitAt bucket entries: one workgroup per tile.
+// One launch, two roles.  The first tot_split workgroups (work table
+// st_split) take the split tiles of the step, which the bead update listed:
+// workgroup j of a replica takes split tiles j, j + G, ... (G = its share);
+// they come first so that the fullest tiles start at once.  The remaining
+// workgroups (work table st_tile) take one tile each: every touched tile, or
+// with R.split those of at most kSplitAt bucket entries.
 template <typename T>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps, int nrep,
-                                                           const long long *__restrict__ start,
-                                                           long long total) {
+                                                           const long long *__restrict__ st_split,
+                                                           long long tot_split,
+                                                           const long long *__restrict__ st_tile,
+                                                           long long tot_tile) {
     __shared__ SpreadShared sh;
     __shared__ SpreadGroup<T> grp[1];
-    const long long g = blockIdx.x;
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
+    const long long gb = blockIdx.x;
+    if (gb < tot_split) {
+        const int r = find_rep(st_split, nrep, gb);
+        const RepDev &R = reps[r];
+        const int j = (int)(gb - st_split[r]), G = (int)(st_split[r + 1] - st_split[r]);
+        const uint32_t step = (uint32_t)(R.ts[0] + 1);
+        const int par = step & 1;
+        const int nsplit = R.ctr[kCtrSplits + par];
+        if (j >= nsplit) return;
+        // issue priority over the thermal forcing on the forked stream
+        __builtin_amdgcn_s_setprio(3);
+        for (int k = j; k < nsplit; k += G) {
+            const int t = gbl(R.slist)[k];
+            spread_tile<T, 1>(R, t, R.bcnt[t], step, par, sh, grp);
+        }
+        return;
+    }
+    const long long g = gb - tot_split;
+    if (g >= tot_tile) return;
+    const int r = find_rep(st_tile, nrep, g);
     const RepDev &R = reps[r];
-    const int t = (int)(g - start[r]);
+    const int t = (int)(g - st_tile[r]);
     const uint32_t step = (uint32_t)(R.ts[0] + 1);
     const int par = step & 1;
     if (t == 0 && threadIdx.x == 0) {
@@ -524,55 +548,11 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
         R.ctr[kCtrSplits + (par ^ 1)] = 0;
     }
     const int cnt = R.bcnt[t];
-    // untouched tiles keep a stale stamp; split tiles are k_spread_split's
-    if (cnt == 0 || cnt > kSplitAt) return;
-    // Issue priority over the thermal forcing that runs beside the spread on
-    // the forked stream: the step waits for the spread, not for the forcing,
-    // which has the first FFT pass to finish in.
+    // untouched tiles keep a stale stamp; split tiles have their own workgroups
+    if (cnt == 0 || (R.split && cnt > kSplitAt)) return;
+    // issue priority over the thermal forcing on the forked stream
     __builtin_amdgcn_s_setprio(3);
     spread_tile<T, 1>(R, t, cnt, step, par, sh, grp);
-}
-
-// Tiles of more than kSplitAt bucket entries, from the split list the bead
-// update filled: four wave-groups add four chunks at once.  Workgroup j of a replica takes
-// split tiles j, j + G, ... (G = its workgroups).  The LDS is dynamic
-// (SpreadShared, then 4 SpreadGroup<T>; more than the 64 KB of static LDS).
-constexpr int kSplitGroups = 4;
-template <typename T>
-__global__ __launch_bounds__(256 * kSplitGroups) void k_spread_split(const RepDev *__restrict__ reps, int nrep,
-                                                                     const long long *__restrict__ start,
-                                                                     long long total) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    SpreadShared &sh = *(SpreadShared *)smem;
-    SpreadGroup<T> *grp = (SpreadGroup<T> *)(smem + sizeof(SpreadShared));
-    const long long g = blockIdx.x;
-    if (g >= total) return;
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const int j = (int)(g - start[r]), G = (int)(start[r + 1] - start[r]);
-    const uint32_t step = (uint32_t)(R.ts[0] + 1);
-    const int par = step & 1;
-    const int nsplit = R.ctr[kCtrSplits + par];
-    if (j >= nsplit) return;
-    __builtin_amdgcn_s_setprio(3);
-    for (int k = j; k < nsplit; k += G) {
-        const int t = gbl(R.slist)[k];
-        spread_tile<T, kSplitGroups>(R, t, R.bcnt[t], step, par, sh, grp);
-    }
-}
-
-size_t spread_split_lds(bool fp64) {
-    return sizeof(SpreadShared) +
-           kSplitGroups * (fp64 ? sizeof(SpreadGroup<double>) : sizeof(SpreadGroup<float>));
-}
-
-hipError_t spread_split_prepare() {
-    hipError_t e = hipFuncSetAttribute((const void *)k_spread_split<float>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)spread_split_lds(false));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_spread_split<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)spread_split_lds(true));
-    return e;
 }
 
 // u* over the whole grid into fsp (runtime-length FFT path only)
@@ -1085,21 +1065,6 @@ static inline unsigned nblk(long long n, int per) { return (unsigned)((n + per -
 template <typename T>
 static void launch_fft_generic(const StepLaunch &L, int stage, hipStream_t st);
 
-// the one-chunk tiles of the spread, and u* on the runtime-length FFT path
-template <typename T>
-static void launch_spread_small_t(const StepLaunch &L, hipStream_t st) {
-    hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)L.tot_tile), dim3(kSpreadThreads), 0, st, L.reps, L.nrep,
-                       L.st_tile, L.tot_tile);
-    if (!L.fast)
-        hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, L.reps, L.nrep, L.st_pts,
-                           L.tot_pts);
-}
-
-static void launch_spread_small(const StepLaunch &L, hipStream_t st) {
-    if (L.fp64) launch_spread_small_t<double>(L, st);
-    else launch_spread_small_t<float>(L, st);
-}
-
 template <typename T>
 static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
     const RepDev *reps = L.reps;
@@ -1114,8 +1079,11 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
                                    st, reps, nrep, L.st_bead, L.tot_bead);
             break;
         case kStageSpread:
-            launch_spread_split(L, st);
-            launch_spread_small_t<T>(L, st);
+            hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)(L.tot_split + L.tot_tile)), dim3(kSpreadThreads), 0,
+                               st, reps, nrep, L.st_split, L.tot_split, L.st_tile, L.tot_tile);
+            if (!L.fast)
+                hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep, L.st_pts,
+                                   L.tot_pts);
             break;
         case kStageFftX:
         case kStageFftY:
@@ -1159,16 +1127,6 @@ static void launch_fft_generic(const StepLaunch &L, int stage, hipStream_t st) {
     }
 }
 
-void launch_spread_split(const StepLaunch &L, hipStream_t st) {
-    if (L.tot_split <= 0) return;
-    if (L.fp64)
-        hipLaunchKernelGGL(k_spread_split<double>, dim3((unsigned)L.tot_split), dim3(256 * kSplitGroups),
-                           spread_split_lds(true), st, L.reps, L.nrep, L.st_split, L.tot_split);
-    else
-        hipLaunchKernelGGL(k_spread_split<float>, dim3((unsigned)L.tot_split), dim3(256 * kSplitGroups),
-                           spread_split_lds(false), st, L.reps, L.nrep, L.st_split, L.tot_split);
-}
-
 void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
     if (L.fp64) launch_stage_t<double>(L, stage, st);
     else launch_stage_t<float>(L, stage, st);
@@ -1184,24 +1142,24 @@ void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
 void launch_step(const StepLaunch &L, hipStream_t st) {
     const bool fork = L.tot_modes > 0;
     launch_stage(L, kStageBead, st);
-    if (fork) {
-        // the forked branch: the split tiles of the spread (joined before the
-        // X pass, which reads the pool), then the thermal forcing and the
-        // conservative forces (joined before the Z pass)
-        (void)hipEventRecord(L.ev_fork, st);
-        (void)hipStreamWaitEvent(L.st2, L.ev_fork, 0);
-        launch_spread_split(L, L.st2);
-        (void)hipEventRecord(L.ev_split, L.st2);
-        launch_noise(L, L.st2);
-        launch_stage(L, kStageBonds, L.st2);
-        (void)hipEventRecord(L.ev_join, L.st2);
+    if (!fork) {
+        for (int s = kStageSpread; s < kNumStages; s++) launch_stage(L, s, st);
+        return;
     }
-    for (int s = kStageSpread; s < kNumStages; s++) {
-        if (fork && s == kStageBonds) continue;
-        if (s == kStageFftX && fork) (void)hipStreamWaitEvent(st, L.ev_split, 0);
-        if (s == kStageFftZ && fork) (void)hipStreamWaitEvent(st, L.ev_join, 0);
-        if (s == kStageSpread && fork) launch_spread_small(L, st);
-        else launch_stage(L, s, st);
+    // The spread is captured before the forked branch's kernels: the graph
+    // keeps the first child of a node on its parent's queue, and a
+    // dependency between queues costs several microseconds, so the main
+    // chain stays on one queue.
+    (void)hipEventRecord(L.ev_fork, st);
+    launch_stage(L, kStageSpread, st);
+    (void)hipStreamWaitEvent(L.st2, L.ev_fork, 0);
+    launch_noise(L, L.st2);
+    launch_stage(L, kStageBonds, L.st2);
+    (void)hipEventRecord(L.ev_join, L.st2);
+    for (int s = kStageFftX; s < kNumStages; s++) {
+        if (s == kStageBonds) continue;
+        if (s == kStageFftZ) (void)hipStreamWaitEvent(st, L.ev_join, 0);
+        launch_stage(L, s, st);
     }
 }
 

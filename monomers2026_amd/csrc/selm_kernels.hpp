@@ -35,20 +35,17 @@ struct StepLaunch {
     bool plane;                      // ... as the three-pass plane path (XYf, Zq, XYi)
     int okmax;                       // plane path: LDS ints for one plane's tile stamps
     int zq_cols;                     // plane path: (kx, ky) columns per Z-pass workgroup (17 or 9)
-    const long long *st_split;       // split-tile work table (k_spread_split)
+    const long long *st_split;       // work table of the spread's split-tile workgroups
     long long tot_split;
     const long long *st_modes;       // thermal-forcing work table (half-spectrum modes)
     long long tot_modes;             // 0 when no replica has thermal forcing (or not fast)
     hipStream_t st2;                 // forked stream for the thermal forcing
     hipEvent_t ev_fork, ev_join;
-    hipEvent_t ev_split;             // the forked split-tile spread is done
 };
 
 void launch_step(const StepLaunch &L, hipStream_t st);
 void launch_stage(const StepLaunch &L, int stage, hipStream_t st);
 void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st);
-void launch_spread_split(const StepLaunch &L, hipStream_t st);
-hipError_t spread_split_prepare();
 void launch_fill_f64(double *p, long long n, double v, hipStream_t st);
 void launch_fill_f32(float *p, long long n, float v, hipStream_t st);
 
