@@ -5,7 +5,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATH = os.path.join(ROOT, "monomers2026_amd", "libselm_host.so")
+# SELM_HOST_LIB selects another build of the library (the sanitizer build of
+# tests/test_sanitizers.py)
+PATH = os.environ.get("SELM_HOST_LIB") or os.path.join(ROOT, "monomers2026_amd", "libselm_host.so")
 DRIVER = os.path.join(ROOT, "monomers2026_amd", "selm_hip_driver")
 DECK_SMALL = os.path.join(ROOT, "tests", "golden", "deck_small")
 

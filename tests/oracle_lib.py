@@ -11,7 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+# SELM_ORACLE_LIB selects another build of the oracle (the sanitizer build of
+# tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("SELM_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 
 class OracleParams(C.Structure):
