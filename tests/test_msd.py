@@ -77,3 +77,52 @@ def test_read_dcd_matches_the_reference_header(tmp_path):
     assert d["frames"].shape == (5, 3, n)
     np.testing.assert_array_equal(d["frames"][0].T, xyz)
     assert tuple(d["cells"][0]) == (6.0, 0.0, 6.0, 0.0, 0.0, 6.0)
+
+
+DCD_1BASE = "/root/reference/data/cases/Diffusion Coefficient/1base.dcd"
+
+
+def _msd_float32_sequential(frames, box_x, nlags):
+    """The script's loop with NumPy >= 2 scalar promotion (NEP 50): a Python
+    int 0 plus a float32 term stays float32, so msd[d][lag] is a float32 sum
+    taken sequentially over atoms (outer) and origins (inner); the terms are
+    formed in float32 (difference, the +-2X shift, the square) and MSD(lag) =
+    (m_x + m_y + m_z) / (Fd * natoms) in float32.  np.cumsum adds in order."""
+    F, _, n = frames.shape
+    X = np.float32(np.float32(box_x) / np.float32(2))
+    start = F // 5
+    fr = frames[start:]
+    Fn = fr.shape[0]
+    out = np.zeros(nlags, np.float64)
+    for lag in range(1, nlags):
+        Fd = Fn - lag
+        m = []
+        for d in range(3):
+            diff = (fr[lag:, d, :] - fr[:Fd, d, :]).astype(np.float32)  # [origin][atom]
+            t = np.where(diff >= X, (diff - np.float32(2) * X) ** 2,
+                         np.where(diff <= -X, (diff + np.float32(2) * X) ** 2, diff ** 2)).astype(np.float32)
+            m.append(np.cumsum(t.T.ravel(), dtype=np.float32)[-1])  # atoms outer, origins inner
+        out[lag] = np.float32(np.float32(m[0] + m[1]) + m[2]) / np.float32(Fd * n)
+    return out
+
+
+def test_msd_accumulation_precision_on_the_reference_trajectory():
+    """The reference's trajectory (cases/Diffusion Coefficient/1base.dcd:
+    5001 frames of 4 beads): the oracle's MSD (float32 terms, FP64 sums, the
+    script's result under NumPy 1.x, where 0 + float32 promotes to float64)
+    against the script's float32 accumulation under NumPy >= 2, for the
+    script's 40 lags.  The difference is the float32 rounding of ~16,000
+    sequential additions per lag; it is printed and bounded here."""
+    import pytest
+    if not os.path.exists(DCD_1BASE):
+        pytest.skip("the reference trajectory is not present")
+    _, _, nat, frames = H.read_dcd(DCD_1BASE)
+    fr = np.stack([xyz.T for _, xyz in frames]).astype(np.float32)  # [F][3][natoms]
+    box = frames[0][0][0]
+    m64 = O.msd(fr, box, 41)
+    m32 = _msd_float32_sequential(fr, box, 41)
+    rel = np.abs(m32[1:] - m64[1:]) / m64[1:]
+    print("1base.dcd MSD, float32 sequential (NumPy >= 2) vs FP64 sums: max rel diff %.2e at lag %d; "
+          "MSD(40) %.6g vs %.6g" % (rel.max(), 1 + int(rel.argmax()), m32[40], m64[40]))
+    assert nat == 4 and fr.shape[0] == 5001
+    assert rel.max() < 1e-4
