@@ -34,6 +34,7 @@ extern "C" {
 #define SELM_ERR_UNSUPPORTED 5  /* grid edge not a product of 2,3,5 in [4, 512] */
 #define SELM_ERR_POOL_FULL 6    /* pool: no slot or not enough budget for this footprint */
 #define SELM_ERR_DEVICE 7       /* a kernel raised a status flag (SELM_STATUS_*); see selm_status */
+#define SELM_ERR_SCRATCH 8      /* request larger than the fixed scratch of selm_msd_noalloc */
 
 /* Device status flags of a system, raised by the step's kernels and kept
  * until cleared with selm_status / selm_pool_status. */
@@ -230,6 +231,24 @@ void *selm_stream(selm_ctx *ctx);
  * Computed on the GPU; terms in FP32 as the script does, sums in FP64. */
 int selm_msd(const float *frames, int nframes, int natoms, const double box[3], int nlags, int mult,
              double *msd);
+
+/* selm_msd without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).
+ * - d_frames, d_msd: DEVICE pointers, the same layouts as selm_msd's frames
+ *   and msd; box is a host array.  Enqueued on stream (NULL: the null
+ *   stream) and not waited for; d_msd holds the result when the stream has
+ *   reached this point.  Results are bitwise those of selm_msd.
+ * - The partial sums live in a fixed device array of 65536 doubles (512 KiB,
+ *   held by each GPU the library runs on until the process ends); a request
+ *   needs at most 2048 + nlags - 1 of them, so nlags <= 65537 always fits
+ *   and a larger nlags returns SELM_ERR_SCRATCH.
+ * - The first call on a device resolves that device's array address (make it
+ *   outside graph capture).  Calls on one GPU share its array: calls that
+ *   overlap in time on one GPU (different streams, or threads) race.
+ * Calls no hipMalloc, hipFree or allocating library routine. */
+int selm_msd_noalloc(const float *d_frames, int nframes, int natoms, const double box[3], int nlags, int mult,
+                     double *d_msd, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Ensembles and serving (no counterpart in the reference, which runs one

@@ -23,6 +23,7 @@ SELM_ERR_NOMEM = 4
 SELM_ERR_UNSUPPORTED = 5
 SELM_ERR_POOL_FULL = 6
 SELM_ERR_DEVICE = 7
+SELM_ERR_SCRATCH = 8
 
 # device status flags (selm_status / selm_pool_status)
 STATUS_TILE_OVERFLOW = 1
@@ -44,7 +45,7 @@ EXPORTS = [
     "selm_pool_step", "selm_pool_sync", "selm_pool_get_beads", "selm_pool_get_field",
     "selm_pool_timestep", "selm_pool_retire", "selm_pool_stats", "selm_pool_stream",
     "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes", "selm_autotune",
-    "selm_pool_autotune", "selm_status", "selm_pool_status", "selm_check_beads",
+    "selm_pool_autotune", "selm_status", "selm_pool_status", "selm_check_beads", "selm_msd_noalloc",
 ]
 
 
@@ -131,6 +132,7 @@ def load(path=None):
     L.selm_pool_stream.argtypes = [vp]
     L.selm_pool_stream.restype = vp
     L.selm_msd.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, dp, C.c_int, C.c_int, dp]
+    L.selm_msd_noalloc.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int, vp, vp]
     L.selm_set_angles.argtypes = [vp, C.c_int, ip, dp, dp]
     L.selm_set_pair_lj.argtypes = [vp, C.c_int, ip, dp, dp, dp, dp]
     _lib = L
