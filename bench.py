@@ -63,23 +63,30 @@ def stage_bytes(w, s, passes=5):
     reads its input once and writes its output once.  Beads: the 4x4x4
     stencil of 3 components (192 values) plus FP64 state.  With the plane
     FFT path (passes == 3) the X and Y transforms of a plane are one pass, so
-    the Y stages move nothing of their own."""
+    the Y stages move nothing of their own; the spread launch also draws the
+    thermal forcing (its 3 Nc complex values written), and the XYi launch
+    evaluates the bond forces (no angle or lj/cut terms here), so those two
+    stages have no kernel of their own."""
     N = w.edge ** 3
     Nc = (w.edge // 2 + 1) * w.edge * w.edge
     M = w.M
-    ypass = 0 if passes == 3 else 2 * 3 * Nc * 2 * s
+    plane = passes == 3
+    ypass = 0 if plane else 2 * 3 * Nc * 2 * s
+    noise = 3 * Nc * 2 * s
+    spread = M * (24 + 3 * s) + 192 * M * s
+    bonds = M * (48 + 24 * 2 + 24)
     return {
-        "thermal_noise": 3 * Nc * 2 * s,
+        "thermal_noise": 0 if plane else noise,
         # stencil values, x read, bead state (x, v, f, m, t_f, s, f_total),
         # one 48-byte bucket entry per (tile, bead) pair
         "bead_update": M * (192 * s + 24 + 152) + 48 * tile_pairs(w),
-        "spread": M * (24 + 3 * s) + 192 * M * s,
+        "spread": spread + (noise if plane else 0),
         "ustar_fft_x": 3 * N * s + 3 * Nc * 2 * s,
         "fft_y": ypass,
         "fft_z_projection": 3 * 3 * Nc * 2 * s,
         "ifft_y": ypass,
-        "ifft_x": 3 * Nc * 2 * s + 3 * N * s,
-        "bond_forces": M * (48 + 24 * 2 + 24),
+        "ifft_x": 3 * Nc * 2 * s + 3 * N * s + (bonds if plane else 0),
+        "bond_forces": 0 if plane else bonds,
     }
 
 

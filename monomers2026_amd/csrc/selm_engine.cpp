@@ -875,6 +875,8 @@ struct Batch {
         L.lj_bins = false;
         for (int r = 0; r < nrep; r++) L.lj_bins = L.lj_bins || reps[r]->D.ljb_n > 0;
         L.plane = plane;
+        L.aux_bonds = plane;
+        for (int r = 0; r < nrep; r++) L.aux_bonds = L.aux_bonds && reps[r]->ang_a.empty() && reps[r]->lj_nt == 0;
         L.okmax = okmax;
         L.zq_cols = zq_cols;
         dirty = false;

@@ -26,6 +26,7 @@
 
 #include "selm_fft.hpp"
 #include "selm_fluid.hpp"
+#include "selm_forces.hpp"
 #include "selm_kernels.hpp"
 
 namespace selm {
@@ -411,7 +412,13 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C>
 //   XYi    P plane z                  iDFT y, c2r along x     u^{n+1} plane z
 //
 // P lives in spec; the thermal forcing nbuf is stored as [d][kz][kx][ky] to
-// match.  Both XY passes keep the plane in one LDS buffer of N x (N/2+1)
+// match.  The step runs as one chain of kernels on one queue: the spread
+// launch draws the thermal forcing in workgroups of its own beside the tiles,
+// and, as a 128^3 grid has 384 plane workgroups (three quarters of the 512
+// that two per CU hold), the XYi launch evaluates the bond forces at x^{n+1}
+// for the next step's bead update in auxiliary workgroups in the free slots
+// (systems without angle or lj/cut terms; the others keep their own kernel).
+// Both XY passes keep the plane in one LDS buffer of N x (N/2+1)
 // elements (element-major, stride N/2+1): the u plane is staged in it, u* is
 // formed into registers and written back as N/2 complex sequences (rows 2p
 // and 2p+1 as real and imaginary parts), transformed along x, unpacked into
@@ -867,36 +874,6 @@ __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const 
 
 // ------------------------------------------------------------------ thermal forcing
 // One mode of the thermal forcing: its three components into nbuf.
-template <typename T>
-__device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int nrep,
-                                           const long long *__restrict__ start, long long g, int plane) {
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    if (!R.noise) return;
-    const long long m = g - start[r];
-    const int ny = R.n[1], nz = R.n[2];
-    const int mi = (int)m;  // Nc < 2^31
-    int i, j, k;
-    if (plane) {
-        const int ki = div_pos(mi, ny, 1.0 / ny);
-        j = mi - ki * ny;
-        k = div_pos(ki, R.xc, 1.0 / R.xc);
-        i = ki - k * R.xc;
-    } else {
-        const int ij = div_pos(mi, nz, 1.0 / nz);
-        k = mi - ij * nz;
-        i = div_pos(ij, ny, 1.0 / ny);
-        j = ij - i * ny;
-    }
-    const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
-    const T lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
-    cplx<T> gv[3];
-    thermal_mode<T>(R, i, j, k, lam, (uint32_t)(R.ts[0] + 1), gv);
-    SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
-#pragma unroll
-    for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
-}
-
 // k_noise: modes in the row order of C ([kx][ky][kz], kz fastest); with plane = 1 in
 // the order of the plane path's P ([kz][kx][ky], ky fastest).  Runs on a
 // forked stream beside the spread and the first FFT pass.  The grid is
@@ -960,15 +937,23 @@ SELM_FFT_KERNEL(k_zq9, 2, SELM_CASE_ZQ9, NTZ9, __attribute__((amdgpu_waves_per_e
 
 // XY passes of the plane path.  Dynamic LDS: the x and y twiddles (twmax
 // slots each), the tile stamps of one plane (okmax ints), then the plane.
-#define SELM_PLANE_KERNEL(kname, BODY, ...)                                                            \
+// Auxiliary workgroups (plane_block_role) stride over aux_total work items
+// (AUX(i) for each) with aux * NT threads.
+#define SELM_PLANE_KERNEL(kname, BODY, AUX)                                                           \
     template <typename T, int NMAX>                                                                   \
     __global__ __launch_bounds__(PlaneNT<NMAX>::v) __attribute__((amdgpu_waves_per_eu(4)))           \
-    void kname(const RepDev *__restrict__ reps, int nrep,                                             \
-                                                             const long long *__restrict__ start,     \
-                                                             long long total, int twmax, int okmax) { \
+    void kname(const RepDev *__restrict__ reps, int nrep, const long long *__restrict__ start,        \
+               long long total, int twmax, int okmax, const long long *__restrict__ aux_start,        \
+               long long aux_total, int aux) {                                                        \
         extern __shared__ __align__(16) unsigned char smem[];                                         \
-        const long long g = xcd_remap(blockIdx.x, gridDim.x);                                         \
-        if (g >= total) return;                                                                       \
+        int ablk;                                                                                     \
+        const long long g = plane_block_role(total, aux, &ablk);                                      \
+        if (g < 0) {                                                                                  \
+            constexpr long long NTA = PlaneNT<NMAX>::v;                                               \
+            if (ablk >= 0)                                                                            \
+                for (long long i = ablk * NTA + threadIdx.x; i < aux_total; i += aux * NTA) AUX(i);   \
+            return;                                                                                   \
+        }                                                                                             \
         const int r = find_rep(start, nrep, g);                                                       \
         const RepDev &R = reps[r];                                                                    \
         const long long item = g - start[r];                                                          \
@@ -989,8 +974,10 @@ SELM_FFT_KERNEL(k_zq9, 2, SELM_CASE_ZQ9, NTZ9, __attribute__((amdgpu_waves_per_e
     case n: if constexpr (n <= NMAX) xyf_body<T, n, PlaneNT<NMAX>::v>(R, item, twx, twy, okf, tab, a); break;
 #define SELM_CASE_XYI(n) \
     case n: if constexpr (n <= NMAX) xyi_body<T, n, PlaneNT<NMAX>::v>(R, item, twx, twy, a); break;
-SELM_PLANE_KERNEL(k_xyf, SELM_CASE_XYF)
-SELM_PLANE_KERNEL(k_xyi, SELM_CASE_XYI)
+#define SELM_AUX_NONE(i) (void)(i)
+#define SELM_AUX_BONDS(i) bond_only_forces(reps, nrep, aux_start, i)
+SELM_PLANE_KERNEL(k_xyf, SELM_CASE_XYF, SELM_AUX_NONE)
+SELM_PLANE_KERNEL(k_xyi, SELM_CASE_XYI, SELM_AUX_BONDS)
 SELM_FFT_KERNEL(k_yf, 1, SELM_CASE_YF, NT)
 SELM_FFT_KERNEL(k_zp, 2, SELM_CASE_Z, NTZ)
 SELM_FFT_KERNEL(k_yi, 1, SELM_CASE_YI, NT)
@@ -1035,6 +1022,14 @@ void fft_fast_lds(bool fp64, const int n[3], int twmax, size_t *lx, size_t *ly, 
 }
 
 static inline unsigned pad8(long long n) { return (unsigned)((n + 7) & ~7LL); }
+
+// Auxiliary workgroups of a plane launch for `work` items at `per` items per
+// thread, at most `cap`.
+static inline int aux_blocks(long long work, int nt, int per, int cap) {
+    if (work <= 0) return 0;
+    const long long b = (work + (long long)nt * per - 1) / ((long long)nt * per);
+    return (int)std::min<long long>(b, cap);
+}
 
 // ---- plane path
 // A replica can take the plane path when nx == ny <= 128 (both compile-time
@@ -1092,8 +1087,8 @@ static void launch_plane_n(const StepLaunch &L, int stage, hipStream_t st) {
     constexpr int NT = PlaneNT<NMAX>::v;
     switch (stage) {
         case kStageFftX:
-            hipLaunchKernelGGL((k_xyf<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep,
-                               L.st_fx, L.tot_fx, L.twmax, L.okmax);
+            hipLaunchKernelGGL((k_xyf<T, NMAX>), dim3(plane_grid(L.tot_fx, 0)), dim3(NT), L.lds_fx, st, L.reps,
+                               L.nrep, L.st_fx, L.tot_fx, L.twmax, L.okmax, nullptr, 0LL, 0);
             break;
         case kStageFftZ:
             if (L.zq_cols == 9)
@@ -1103,10 +1098,13 @@ static void launch_plane_n(const StepLaunch &L, int stage, hipStream_t st) {
                 hipLaunchKernelGGL((k_zq<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
                                    L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
             break;
-        case kStageFftXi:
-            hipLaunchKernelGGL((k_xyi<T, NMAX>), dim3(pad8(L.tot_fx)), dim3(NT), L.lds_fx, st, L.reps, L.nrep,
-                               L.st_fx, L.tot_fx, L.twmax, L.okmax);
+        case kStageFftXi: {
+            const long long nb = L.aux_bonds ? L.tot_bead : 0;
+            const int aux = aux_blocks(nb, NT, 1, 1 << 20);
+            hipLaunchKernelGGL((k_xyi<T, NMAX>), dim3(plane_grid(L.tot_fx, aux)), dim3(NT), L.lds_fx, st, L.reps,
+                               L.nrep, L.st_fx, L.tot_fx, L.twmax, L.okmax, L.st_bead, nb, aux);
             break;
+        }
         default: break;  // the Y passes are folded into the XY passes
     }
 }
@@ -1154,7 +1152,7 @@ static void launch_fast_t(const StepLaunch &L, int stage, hipStream_t st) {
 }
 
 void launch_noise(const StepLaunch &L, hipStream_t st) {
-    if (L.tot_modes <= 0) return;
+    if (L.tot_modes <= 0 || L.plane) return;  // the plane path's spread launch draws the forcing
     const unsigned nb = (unsigned)std::min<long long>((L.tot_modes + 255) / 256, kNoiseBlocks);
     const int plane = L.plane ? 1 : 0;
     if (L.fp64)

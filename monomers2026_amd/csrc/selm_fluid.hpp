@@ -87,4 +87,37 @@ __device__ __forceinline__ void thermal_mode(const RepDev &R, int i, int j, int 
     }
 }
 
+// Thermal forcing of mode g of a batch's work table (modes of replica r at
+// start[r] ..) into nbuf: the row order of C ([kx][ky][kz], kz fastest), or
+// with plane = 1 that of the plane path's P ([kz][kx][ky], ky fastest).
+template <typename T>
+__device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int nrep,
+                                           const long long *__restrict__ start, long long g, int plane) {
+    const int r = find_rep(start, nrep, g);
+    const RepDev &R = reps[r];
+    if (!R.noise) return;
+    const long long m = g - start[r];
+    const int ny = R.n[1], nz = R.n[2];
+    const int mi = (int)m;  // Nc < 2^31
+    int i, j, k;
+    if (plane) {
+        const int ki = div_pos(mi, ny, 1.0 / ny);
+        j = mi - ki * ny;
+        k = div_pos(ki, R.xc, 1.0 / R.xc);
+        i = ki - k * R.xc;
+    } else {
+        const int ij = div_pos(mi, nz, 1.0 / nz);
+        k = mi - ij * nz;
+        i = div_pos(ij, ny, 1.0 / ny);
+        j = ij - i * ny;
+    }
+    const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
+    const T lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
+    cplx<T> gv[3];
+    thermal_mode<T>(R, i, j, k, lam, (uint32_t)(R.ts[0] + 1), gv);
+    SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
+#pragma unroll
+    for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
+}
+
 }  // namespace selm

@@ -14,7 +14,8 @@
 //                     (bead, component) added by one wave (lanes = stencil
 //                     points, distinct cells) as 64-bit fixed point, so the
 //                     sums do not depend on the order; the tile goes to the
-//                     sparse pool, untouched tiles are marked empty
+//                     sparse pool, untouched tiles are marked empty.  On the
+//                     plane FFT path the same launch draws the thermal forcing
 //   [k_ustar]         u* = u + (mu dt L7 u + f) / rho (runtime-length FFT path
 //                     only; the compile-time-length X pass computes u* itself)
 //   X, Y, Z+projection, Y, X passes of the 3-D FFT (selm_fft_kernels.hip for
@@ -23,8 +24,12 @@
 //                     (only when a replica has them)
 //   k_bond_forces     thread per bead: harmonic bonds and angles, lj/cut pairs
 //                     and the external force at x^{n+1}; advances each
-//                     replica's step counter
+//                     replica's step counter (on the plane FFT path without
+//                     angle or lj/cut terms the XYi launch does this)
+#include <algorithm>
+
 #include "selm_fluid.hpp"
+#include "selm_forces.hpp"
 #include "selm_kernels.hpp"
 
 namespace selm {
@@ -218,6 +223,21 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
 // ===========================================================================
 constexpr int kTileCells = 512;   // tile edge <= 8
 constexpr int kSpreadThreads = 256;
+// Thermal-forcing workgroups of a spread launch: about 8 modes per thread,
+// in whole multiples of the 256 CUs (blocks are dealt round-robin, and a CU
+// with one forcing workgroup more than the others finishes last), at most
+// kSpreadNoiseMax.  Timed at 128^3, 20k beads (1.06e6 modes): 128, 256, 512,
+// 768, 1024, 2048 workgroups gave 0.174, 0.149, 0.142, 0.145, 0.145, 0.146 ms
+// per step; 520 gave 0.147.  On the 256-system ensemble (1.1e7 modes) a cap
+// of 512, 1024, 2048, 8192 gave 1.148, 1.093, 1.105, 1.112 ms per step.
+constexpr int kSpreadModesPerThread = 8;
+constexpr int kSpreadNoiseMax = 1024;
+static inline int spread_noise_blocks(long long modes) {
+    if (modes <= 0) return 0;
+    long long b = modes / ((long long)kSpreadThreads * kSpreadModesPerThread);
+    b = b / 256 * 256;
+    return (int)std::min<long long>(std::max<long long>(b, 256), kSpreadNoiseMax);
+}
 
 // round(v 2^S) as a 64-bit integer for |v 2^S| < 2^51 (scale = 2^S): adding
 // 1.5 2^52 leaves the rounded integer in the low mantissa bits.
@@ -504,21 +524,35 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     }
 }
 
-// One launch, two roles.  The first tot_split workgroups (work table
-// st_split) take the split tiles of the step, which the bead update listed:
-// workgroup j of a replica takes split tiles j, j + G, ... (G = its share);
-// they come first so that the fullest tiles start at once.  The remaining
-// workgroups (work table st_tile) take one tile each: every touched tile, or
-// with R.split those of at most kSplitAt bucket entries.
+// One launch, three roles.  The first nnoise workgroups draw the thermal
+// forcing of the step into nbuf (plane path; work table st_modes, each
+// thread striding over the modes): about two workgroups per CU for most of
+// the launch, at the default issue priority, so the spread's VALU-light LDS
+// work hides much of the forcing's arithmetic.  The next tot_split workgroups (work
+// table st_split) take the split tiles of the step, which the bead update
+// listed: workgroup j of a replica takes split tiles j, j + G, ... (G = its
+// share); they come early so that the fullest tiles start at once.  The
+// remaining workgroups (work table st_tile) take one tile each: every touched
+// tile, or with R.split those of at most kSplitAt bucket entries.
 template <typename T>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps, int nrep,
+                                                           const long long *__restrict__ st_modes,
+                                                           long long tot_modes, int nnoise,
                                                            const long long *__restrict__ st_split,
                                                            long long tot_split,
                                                            const long long *__restrict__ st_tile,
                                                            long long tot_tile) {
     __shared__ SpreadShared sh;
     __shared__ SpreadGroup<T> grp[1];
-    const long long gb = blockIdx.x;
+    if constexpr (sizeof(T) == 4) {  // the plane path is FP32-bulk; FP64 launches have nnoise = 0
+        if ((int)blockIdx.x < nnoise) {
+            for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes;
+                 m += (long long)nnoise * kSpreadThreads)
+                noise_mode<T>(reps, nrep, st_modes, m, 1);
+            return;
+        }
+    }
+    const long long gb = blockIdx.x - nnoise;
     if (gb < tot_split) {
         const int r = find_rep(st_split, nrep, gb);
         const RepDev &R = reps[r];
@@ -527,7 +561,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
         const int par = step & 1;
         const int nsplit = R.ctr[kCtrSplits + par];
         if (j >= nsplit) return;
-        // issue priority over the thermal forcing on the forked stream
+        // issue priority over the thermal forcing
         __builtin_amdgcn_s_setprio(3);
         for (int k = j; k < nsplit; k += G) {
             const int t = gbl(R.slist)[k];
@@ -550,7 +584,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
     const int cnt = R.bcnt[t];
     // untouched tiles keep a stale stamp; split tiles have their own workgroups
     if (cnt == 0 || (R.split && cnt > kSplitAt)) return;
-    // issue priority over the thermal forcing on the forked stream
+    // issue priority over the thermal forcing
     __builtin_amdgcn_s_setprio(3);
     spread_tile<T, 1>(R, t, cnt, step, par, sh, grp);
 }
@@ -944,30 +978,7 @@ __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ 
     const RepDev &R = reps[r];
     const int b = (int)(g - start[r]);
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int q = R.adj_ptr[b]; q < R.adj_ptr[b + 1]; q++) {
-        const int i1 = R.adj_i1[q], i2 = R.adj_i2[q];
-        double del[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double dd = R.x[3 * i1 + d] - R.x[3 * i2 + d];
-            if (fabs(dd) > 0.5 * R.Lb[d]) dd += (dd < 0.0) ? R.Lb[d] : -R.Lb[d];
-            del[d] = dd;
-        }
-        double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
-        double rr = sqrt(rsq);
-        double dr = rr - R.adj_r0[q];
-        double rk = R.adj_K[q] * dr;
-        double fbond = (rr > 0.0) ? -2.0 * rk / rr : 0.0;
-        if (b == i1) {
-            f0 += del[0] * fbond;
-            f1 += del[1] * fbond;
-            f2 += del[2] * fbond;
-        } else {
-            f0 -= del[0] * fbond;
-            f1 -= del[1] * fbond;
-            f2 -= del[2] * fbond;
-        }
-    }
+    bond_terms(R, b, f0, f1, f2);
     // harmonic angles (LAMMPS angle_style harmonic): E = K (theta - theta0)^2
     // at the centre bead; -dE/dx = (2 K dtheta / sin theta) dcos/dx, with sin
     // theta held away from 0 by 0.001 as LAMMPS does.
@@ -1078,13 +1089,17 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
                 hipLaunchKernelGGL(k_bead_update<T>, dim3(nblk(L.tot_bead, kBeadsPerBlock)), dim3(kBeadThreads), 0,
                                    st, reps, nrep, L.st_bead, L.tot_bead);
             break;
-        case kStageSpread:
-            hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)(L.tot_split + L.tot_tile)), dim3(kSpreadThreads), 0,
-                               st, reps, nrep, L.st_split, L.tot_split, L.st_tile, L.tot_tile);
+        case kStageSpread: {
+            // the plane path's thermal forcing (FP32-bulk only)
+            const int nnoise = L.plane ? spread_noise_blocks(L.tot_modes) : 0;
+            hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)(nnoise + L.tot_split + L.tot_tile)),
+                               dim3(kSpreadThreads), 0, st, reps, nrep, L.st_modes, L.tot_modes, nnoise, L.st_split,
+                               L.tot_split, L.st_tile, L.tot_tile);
             if (!L.fast)
                 hipLaunchKernelGGL(k_ustar<T>, dim3(nblk(L.tot_pts, 256)), dim3(256), 0, st, reps, nrep, L.st_pts,
                                    L.tot_pts);
             break;
+        }
         case kStageFftX:
         case kStageFftY:
         case kStageFftZ:
@@ -1095,7 +1110,7 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
             break;
         case kStageBonds:
             // on the compile-time-length path the Xi pass advances the step counter
-            launch_bond_forces(L, L.fast ? 0 : 1, st);
+            if (!L.aux_bonds) launch_bond_forces(L, L.fast ? 0 : 1, st);
             break;
         default: break;
     }
@@ -1132,15 +1147,18 @@ void launch_stage(const StepLaunch &L, int stage, hipStream_t st) {
     else launch_stage_t<float>(L, stage, st);
 }
 
-// One step.  On the compile-time-length path the thermal forcing and the
-// conservative forces at x^{n+1} run on a forked stream once the bead update
-// is done, concurrently with the spread and the first two FFT passes; the
-// branch joins before the Z pass, which consumes the forcing.  The bead
+// One step.  On the five-pass compile-time-length path the thermal forcing
+// and the conservative forces at x^{n+1} run on a forked stream once the bead
+// update is done, concurrently with the spread and the first two FFT passes;
+// the branch joins before the Z pass, which consumes the forcing.  The bead
 // update runs alone: it is latency-bound and slows down beside the forcing.
-// There the step counter is advanced by the last pass (Xi), after every
-// kernel that reads it; otherwise by the bond kernel at the end of the step.
+// The plane path runs every kernel in one chain on one queue: its spread
+// launch draws the thermal forcing, and its XYi launch (without angle or
+// lj/cut terms) evaluates the bond forces, in workgroups of their own.  On the compile-time-length paths the step counter is
+// advanced by the last pass (Xi or XYi), after every kernel that reads it;
+// otherwise by the bond kernel at the end of the step.
 void launch_step(const StepLaunch &L, hipStream_t st) {
-    const bool fork = L.tot_modes > 0;
+    const bool fork = L.tot_modes > 0 && !L.plane;
     launch_stage(L, kStageBead, st);
     if (!fork) {
         for (int s = kStageSpread; s < kNumStages; s++) launch_stage(L, s, st);

@@ -35,11 +35,13 @@ struct StepLaunch {
     bool plane;                      // ... as the three-pass plane path (XYf, Zq, XYi)
     int okmax;                       // plane path: LDS ints for one plane's tile stamps
     int zq_cols;                     // plane path: (kx, ky) columns per Z-pass workgroup (17 or 9)
+    bool aux_bonds;                  // plane path: the XYi launch evaluates the bond forces (no
+                                     // replica has angle or lj/cut terms); no bond stage of its own
     const long long *st_split;       // work table of the spread's split-tile workgroups
     long long tot_split;
     const long long *st_modes;       // thermal-forcing work table (half-spectrum modes)
     long long tot_modes;             // 0 when no replica has thermal forcing (or not fast)
-    hipStream_t st2;                 // forked stream for the thermal forcing
+    hipStream_t st2;                 // forked stream for the thermal forcing (five-pass path)
     hipEvent_t ev_fork, ev_join;
 };
 
