@@ -182,6 +182,15 @@ int selm_set_field(selm_ctx *ctx, int comp, const double *u);
 /* Number of completed steps (Selm::timestep, cuda/SELM.cuh:11). */
 long long selm_timestep(selm_ctx *ctx);
 
+/* Set the number of completed steps to n (0 <= n <= 2^31 - 2), after
+ * selm_init: the next step is n + 1 and its thermal draws are keyed by n + 1.
+ * A system rebuilt from a snapshot of x and v (selm_set_beads, selm_init),
+ * u (selm_set_field) and n therefore continues bit for bit as the run the
+ * snapshot was taken from.  Replaces Selm::timestep (cuda/SELM.cuh:11) as a
+ * restart point; the reference itself has no restart (`variable restart` is
+ * unused, selm/test/NucleotideMonomer/Model.LAMMPS_script:3). */
+int selm_set_timestep(selm_ctx *ctx, long long n);
+
 /* Device bytes one system with these parameters, M beads and nb bonds holds. */
 int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
 
@@ -288,6 +297,10 @@ int selm_pool_get_beads(selm_pool *pool, int slot, double *x, double *v, double 
                         double *f_total);
 int selm_pool_get_field(selm_pool *pool, int slot, int comp, double *u);
 long long selm_pool_timestep(selm_pool *pool, int slot);
+/* selm_set_field and selm_set_timestep for one admitted system (resume a
+ * snapshot inside a pool). */
+int selm_pool_set_field(selm_pool *pool, int slot, int comp, const double *u);
+int selm_pool_set_timestep(selm_pool *pool, int slot, long long n);
 int selm_pool_retire(selm_pool *pool, int slot);
 /* used/budget bytes of the arena, number of admitted systems */
 /* selm_autotune for a pool: the shapes are timed on the pool's current

@@ -25,6 +25,9 @@ static_assert(kStatusTileOverflow == SELM_STATUS_TILE_OVERFLOW && kStatusNonFini
 
 constexpr size_t kLdsMax = 65536;
 constexpr double kPi = 3.14159265358979323846;
+// the step number keys the thermal draws as 32 bits and stamps the spread
+// tiles as an int
+constexpr long long kMaxTimestep = (1LL << 31) - 2;
 
 #define HIPCHK(expr)                                                                  \
     do {                                                                              \
@@ -989,6 +992,45 @@ static int download_field(Replica &R, hipStream_t st, int comp, double *out, std
     return SELM_OK;
 }
 
+static int upload_field(Replica &R, hipStream_t st, int comp, const double *u, std::string &err) {
+    if (comp < 0 || comp > 2 || !u) {
+        err = "set_field: bad arguments";
+        return SELM_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    const size_t N = (size_t)R.G.N;
+    if (R.fp64) {
+        HIPCHK(hipMemcpy((double *)R.D.u + comp * N, u, N * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(u, u + N);
+        HIPCHK(hipMemcpy((float *)R.D.u + comp * N, tmp.data(), N * 4, hipMemcpyHostToDevice));
+    }
+    return SELM_OK;
+}
+
+// Set the count of completed steps to n, which keys the thermal draws of the
+// next step (n + 1).  The per-step bookkeeping of the spread is reset with
+// it: tile stamps left by an earlier run at a step number the system will
+// reach again would otherwise read as current, and the overflow counters are
+// indexed by the step's parity.
+static int replica_set_timestep(Replica &R, hipStream_t st, long long n, std::string &err) {
+    if (n < 0 || n > kMaxTimestep) {
+        err = "set_timestep: n = " + std::to_string(n) + " outside [0, " + std::to_string(kMaxTimestep) + "]";
+        return SELM_ERR_ARG;
+    }
+    if (!R.inited) {
+        err = "set_timestep before init";
+        return SELM_ERR_STATE;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemsetAsync(R.D.tstamp, 0xff, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.bcnt, 0, (size_t)R.G.ntiles * 4, st));
+    HIPCHK(hipMemsetAsync(R.D.ctr, 0, 8 * 4, st));
+    HIPCHK(hipMemcpyAsync(R.D.ts, &n, 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SELM_OK;
+}
+
 }  // namespace selm
 
 using namespace selm;
@@ -1370,15 +1412,19 @@ int selm_set_field(selm_ctx *c, int comp, const double *u) {
     CTX_CHECK(c);
     std::string &err = c->err;
     if (comp < 0 || comp > 2 || !u) return SELM_ERR_ARG;
-    HIPCHK(hipStreamSynchronize(c->last));
-    const size_t N = (size_t)c->rep.G.N;
-    if (c->rep.fp64) {
-        HIPCHK(hipMemcpy((double *)c->rep.D.u + comp * N, u, N * 8, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> tmp(u, u + N);
-        HIPCHK(hipMemcpy((float *)c->rep.D.u + comp * N, tmp.data(), N * 4, hipMemcpyHostToDevice));
+    if (!c->rep.D.u) {
+        err = "selm_set_field before the grid is allocated (selm_set_beads)";
+        return SELM_ERR_STATE;
     }
-    return SELM_OK;
+    HIPCHK(hipStreamSynchronize(c->last));
+    return upload_field(c->rep, c->last, comp, u, err);
+}
+
+int selm_set_timestep(selm_ctx *c, long long n) {
+    CTX_CHECK(c);
+    std::string &err = c->err;
+    HIPCHK(hipStreamSynchronize(c->last));
+    return replica_set_timestep(c->rep, c->last, n, err);
 }
 
 long long selm_timestep(selm_ctx *c) {
@@ -1696,6 +1742,18 @@ long long selm_pool_timestep(selm_pool *P, int slot) {
     if (hipStreamSynchronize(P->batch.stream) != hipSuccess) return -1;
     if (hipMemcpy(&ts, R->D.ts, 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return ts;
+}
+
+int selm_pool_set_field(selm_pool *P, int slot, int comp, const double *u) {
+    Replica *R = pool_slot(P, slot);
+    if (!R) return SELM_ERR_ARG;
+    return upload_field(*R, P->batch.stream, comp, u, P->err);
+}
+
+int selm_pool_set_timestep(selm_pool *P, int slot, long long n) {
+    Replica *R = pool_slot(P, slot);
+    if (!R) return SELM_ERR_ARG;
+    return replica_set_timestep(*R, P->batch.stream, n, P->err);
 }
 
 int selm_pool_retire(selm_pool *P, int slot) {

@@ -46,6 +46,7 @@ EXPORTS = [
     "selm_pool_timestep", "selm_pool_retire", "selm_pool_stats", "selm_pool_stream",
     "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes", "selm_autotune",
     "selm_pool_autotune", "selm_status", "selm_pool_status", "selm_check_beads", "selm_msd_noalloc",
+    "selm_set_timestep", "selm_pool_set_field", "selm_pool_set_timestep",
 ]
 
 
@@ -127,6 +128,9 @@ def load(path=None):
     L.selm_pool_get_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.selm_pool_timestep.argtypes = [vp, C.c_int]
     L.selm_pool_timestep.restype = C.c_longlong
+    L.selm_set_timestep.argtypes = [vp, C.c_longlong]
+    L.selm_pool_set_field.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.selm_pool_set_timestep.argtypes = [vp, C.c_int, C.c_longlong]
     L.selm_pool_retire.argtypes = [vp, C.c_int]
     L.selm_pool_stats.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), ip]
     L.selm_pool_stream.argtypes = [vp]
@@ -295,6 +299,10 @@ class System:
     def timestep(self):
         return self.L.selm_timestep(self.h)
 
+    def set_timestep(self, n):
+        """selm_set_timestep: resume at n completed steps (after init)."""
+        self._chk(self.L.selm_set_timestep(self.h, int(n)), "set_timestep")
+
     def stage_times(self, nsteps):
         ms = np.zeros(len(STAGES))
         self._chk(self.L.selm_stage_times(self.h, int(nsteps), _d(ms), len(STAGES)), "stage_times")
@@ -406,6 +414,16 @@ class Pool:
 
     def timestep(self, slot):
         return self.L.selm_pool_timestep(self.h, slot)
+
+    def set_field(self, slot, u):
+        _, shape = self.meta[slot]
+        u = f64(u, (3,) + shape)
+        for c in range(3):
+            buf = np.ascontiguousarray(u[c])
+            self._chk(self.L.selm_pool_set_field(self.h, slot, c, _d(buf)), "pool_set_field")
+
+    def set_timestep(self, slot, n):
+        self._chk(self.L.selm_pool_set_timestep(self.h, slot, int(n)), "pool_set_timestep")
 
     def retire(self, slot):
         self._chk(self.L.selm_pool_retire(self.h, slot), "pool_retire")

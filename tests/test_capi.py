@@ -88,3 +88,13 @@ def test_msd_noalloc_rejects_requests_beyond_its_scratch():
     assert L.selm_msd_noalloc(fake, 100000, 4, box, 1 << 20, 1, fake, None) == selm.SELM_ERR_SCRATCH
     assert L.selm_msd_noalloc(None, 100, 4, box, 10, 1, fake, None) == selm.SELM_ERR_ARG
     assert L.selm_msd_noalloc(fake, 100, 4, box, 10, 0, fake, None) == selm.SELM_ERR_ARG
+
+
+def test_resume_entry_points_reject_null_handles():
+    """selm_set_timestep / selm_pool_set_field / selm_pool_set_timestep return
+    SELM_ERR_ARG for a null context or pool without touching a device."""
+    from monomers2026_amd import selm
+    L = selm.load()
+    assert L.selm_set_timestep(None, 0) == selm.SELM_ERR_ARG
+    assert L.selm_pool_set_timestep(None, 0, 0) == selm.SELM_ERR_ARG
+    assert L.selm_pool_set_field(None, 0, 0, None) == selm.SELM_ERR_ARG
