@@ -111,17 +111,19 @@ def reference_stage_bytes(w, s):
 # (--fp32-only --copy-calibration 256), each scaled by what the counter reads
 # for a copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the
 # per-kernel table is profiles/pmc_traffic_dilute20k_128.md.  The step runs
-# the plane FFT path (k_xyf, k_zq9, k_xyi): the Y stages have no kernel.
+# the plane FFT path (k_xyf, k_zq, k_xyi): the Y stages have no kernel, the
+# spread launch draws the thermal forcing and the XYi launch evaluates the
+# bond forces.
 PMC_TRAFFIC_FP32 = {
-    "thermal_noise": 25744384,
-    "bead_update": 36830848,
-    "spread": 18040480,
-    "ustar_fft_x": 89878656,
+    "thermal_noise": 0,
+    "bead_update": 31782502,
+    "spread": 40279347,
+    "ustar_fft_x": 89966387,
     "fft_y": 0,
-    "fft_z_projection": 77985920,
+    "fft_z_projection": 77982515,
     "ifft_y": 0,
-    "ifft_x": 51044640,
-    "bond_forces": 2710272,
+    "ifft_x": 53571482,
+    "bond_forces": 0,
 }
 COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
 
