@@ -1561,6 +1561,11 @@ struct ArenaAlloc {
         for (auto &kv : free_) f += kv.second;
         return size - f;
     }
+    // after compaction: everything below `end` is taken, the rest is free
+    void reset_to(size_t end) {
+        free_.clear();
+        if (end < size) free_[end] = size - end;
+    }
 };
 }  // namespace selm
 
@@ -1572,6 +1577,7 @@ struct selm_pool {
     std::vector<size_t> slot_off, slot_bytes;
     std::string err;
     int precision = 0;
+    long long compactions = 0;  // arena compactions so far
 };
 
 extern "C" {
@@ -1626,6 +1632,43 @@ static void pool_rebuild_list(selm_pool *P) {
     P->batch.dirty = true;
 }
 
+// Move every admitted system down to the start of the arena, in offset
+// order, so that the free space becomes one block.  A system's device state
+// is addressed only through the pointers that carve_grid/carve_beads derive
+// from its base, so a move is a device-to-device copy of its block and a new
+// carve; the batch's descriptors are uploaded again before the next step.  A
+// block that moves by less than its size is copied in pieces no longer than
+// the distance moved, from low to high addresses, so no piece overwrites
+// bytes that a later piece still has to read.
+static int pool_compact(selm_pool *P, std::string &err) {
+    hipStream_t st = P->batch.stream;
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int> order;
+    for (size_t q = 0; q < P->slots.size(); q++)
+        if (P->slots[q]) order.push_back((int)q);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return P->slot_off[a] < P->slot_off[b]; });
+    size_t end = 0;
+    for (int q : order) {
+        const size_t off = P->slot_off[q], bytes = al(P->slot_bytes[q]);
+        if (off != end) {
+            const size_t dist = off - end;
+            for (size_t c = 0; c < bytes; c += dist)
+                HIPCHK(hipMemcpyAsync(P->arena + end + c, P->arena + off + c, std::min(dist, bytes - c),
+                                      hipMemcpyDeviceToDevice, st));
+            Replica &R = *P->slots[q];
+            carve_grid(R, P->arena + end);
+            carve_beads(R, P->arena + end + al(R.grid_sz));
+            P->slot_off[q] = end;
+        }
+        end += bytes;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    P->alloc.reset_to(end);
+    pool_rebuild_list(P);
+    P->compactions++;
+    return SELM_OK;
+}
+
 int selm_pool_admit(selm_pool *P, const selm_params *p, int M, const double *x, const double *v,
                     const double *m, int nb, const int *ij, const double *K, const double *r0, int next,
                     const int *idx, const double *F, int *slot) {
@@ -1656,12 +1699,21 @@ int selm_pool_admit(selm_pool *P, const selm_params *p, int M, const double *x, 
     }
     size_t gb = grid_bytes(R->G, fp64), bb = bead_bytes(R->M, (int)R->adj_i1.size(), fp64);
     size_t off = 0;
-    if (!P->alloc.alloc(gb + bb, off)) {
+    bool ok = P->alloc.alloc(gb + bb, off);
+    if (!ok && P->alloc.size - P->alloc.used() >= al(gb + bb)) {
+        // enough free space in total, but not in one block
+        rc = pool_compact(P, err);
+        if (rc) {
+            delete R;
+            return rc;
+        }
+        ok = P->alloc.alloc(gb + bb, off);
+    }
+    if (!ok) {
         err = "not enough free arena space for footprint " + std::to_string(gb + bb);
         delete R;
         return SELM_ERR_POOL_FULL;
     }
-    HIPCHK(hipStreamSynchronize(P->batch.stream));
     carve_grid(*R, P->arena + off);
     carve_beads(*R, P->arena + off + al(gb));
     R->D.status = P->batch.d_status + s;

@@ -93,3 +93,81 @@ def test_pool_admission_by_footprint(gpu_lib):
     pool.step(3)
     assert pool.timestep(b) == 3
     pool.close()
+
+
+def test_pool_compaction_on_fragmented_arena(gpu_lib):
+    """Three equal systems (40^3) fill the arena; retiring the first and the
+    third leaves two free blocks that together, but not alone, hold a larger
+    system (48^3).  Its admission compacts the arena (the middle system moves to
+    the start) and succeeds; the moved system and the new one then give the
+    same bits as running them alone."""
+    from monomers2026_amd import selm, workloads
+    P = selm.PREC_FP32_BULK
+    rng = np.random.default_rng(21)
+    x, ij = workloads.chains(rng, 1, 64, 40)
+    wa = workloads.Workload("a", 40, x, ij)
+    fa = selm.footprint(wa.params(P, True), wa.M, wa.ij.shape[0])
+    al = lambda b: (b + 255) // 256 * 256
+    wb = None
+    # edges with a compile-time FFT length, so the batch keeps the plane
+    # path that each system runs alone
+    for L in (48, 60):
+        xb, ijb = workloads.chains(rng, 2, 50, L)
+        cand = workloads.Workload("b", L, xb, ijb)
+        fb = selm.footprint(cand.params(P, True), cand.M, cand.ij.shape[0])
+        if al(fa) < al(fb) <= 2 * al(fa):
+            wb = cand
+            break
+    assert wb is not None, "no grid between one and two footprints"
+    pool = selm.Pool(3 * al(fa), 4, precision=P)
+    s1 = _admit(pool, wa, P, 101)
+    s2 = _admit(pool, wa, P, 102)
+    s3 = _admit(pool, wa, P, 103)
+    assert None not in (s1, s2, s3)
+    assert _admit(pool, wa, P, 104) is None  # full
+    pool.step(4)
+    pool.retire(s1)
+    pool.retire(s3)
+    sb = _admit(pool, wb, P, 105)
+    assert sb is not None, "admission after compaction failed"
+    pool.step(6)
+    x2, v2, _, _ = pool.beads(s2)
+    assert pool.timestep(s2) == 10 and pool.timestep(sb) == 6
+    xa, va, ua = _alone(wa, P, 10, 102)
+    assert np.array_equal(x2, xa) and np.array_equal(v2, va)
+    assert np.array_equal(pool.field(s2), ua)
+    xbp, vbp, _, _ = pool.beads(sb)
+    xba, vba, uba = _alone(wb, P, 6, 105)
+    assert np.array_equal(xbp, xba) and np.array_equal(vbp, vba)
+    assert np.array_equal(pool.field(sb), uba)
+    assert pool.status(s2) == 0 and pool.status(sb) == 0
+    print("compaction: moved system and new system bitwise equal to alone", flush=True)
+    pool.close()
+
+
+def test_pool_mixed_fft_path_agrees_to_fp32_rounding(gpu_lib):
+    """A batch takes one FFT path for all of its systems: a 32^3 system
+    (plane path alone) batched with a 36^3 one (36 has no compile-time
+    length) runs the runtime-radix kernels.  It then agrees with its run
+    alone to FP32 rounding, not bit for bit: bead positions within 1e-7 of
+    the box and the field within 1e-5 of its largest value after 10 steps."""
+    from monomers2026_amd import selm, workloads
+    P = selm.PREC_FP32_BULK
+    rng = np.random.default_rng(21)
+    x, ij = workloads.chains(rng, 1, 64, 32)
+    wa = workloads.Workload("a", 32, x, ij)
+    xb, ijb = workloads.chains(rng, 2, 50, 36)
+    wb = workloads.Workload("b", 36, xb, ijb)
+    pool = selm.Pool(1 << 30, 4, precision=P)
+    sa = _admit(pool, wa, P, 102)
+    sb = _admit(pool, wb, P, 105)
+    pool.step(10)
+    xa, va, ua = _alone(wa, P, 10, 102)
+    x2, _, _, _ = pool.beads(sa)
+    u2 = pool.field(sa)
+    ex = np.abs(x2 - xa).max() / 32.0
+    eu = np.abs(u2 - ua).max() / np.abs(ua).max()
+    print("32^3 beside 36^3 vs alone: x %.2e (of the box) u %.2e" % (ex, eu), flush=True)
+    assert ex <= 1e-7 and eu <= 1e-5
+    assert pool.status(sa) == 0 and pool.status(sb) == 0
+    pool.close()
