@@ -202,18 +202,29 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes);
  *  on a forked stream beside stages 2-4; here every stage runs alone. */
 int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
 
-/* Autotuner of the FFT pass shapes (no reference counterpart: the
- * reference's cuFFT plans choose their own, cuda/SELM_Eulerian_Period.cu:26-30).
- * Times `steps` steps of the system under each shape this build has for its
- * grid -- FP32-bulk with nx == ny <= 128: the three-pass plane path with 17 or
- * 9 (kx, ky) columns per Z-pass workgroup, and the five-pass transposing path;
- * one shape otherwise -- restoring the system's device state after each, and
- * keeps the fastest.  The choice is cached per (precision, grid shapes) for
- * the process, and a cached choice is applied without timing.
- * shape[0] = FFT passes (3, 5; 0 for the runtime-length kernels), shape[1] =
- * Z-pass columns per workgroup on the plane path (else 0), *ms = the best
- * milliseconds per step (0 when taken from the cache). */
+/* Autotuner of the FFT pass shapes and of the spread launch's shape (no
+ * reference counterpart: the reference's cuFFT plans choose their own,
+ * cuda/SELM_Eulerian_Period.cu:26-30, and its spread is one thread per bead,
+ * cuda/SELM_Kernel.cu:424-470).  Times `steps` steps of the system under each
+ * FFT shape this build has for its grid -- FP32-bulk with nx == ny <= 128:
+ * the three-pass plane path with 17 or 9 (kx, ky) columns per Z-pass
+ * workgroup, and the five-pass transposing path; one shape otherwise -- and
+ * keeps the fastest; then, with that shape, each spread shape: split
+ * threshold 32, 64, 128 or none (tiles with more bucket entries are taken
+ * first by workgroups of their own) and, on the plane path, half, the
+ * default or twice the thermal-forcing workgroups of the spread launch.  The
+ * system's device state is restored after each candidate.  The choice is
+ * cached per (precision, grid shapes, bead counts) for the process, and a
+ * cached choice is applied without timing.  shape[0] = FFT passes (3, 5; 0
+ * for the runtime-length kernels), shape[1] = Z-pass columns per workgroup on
+ * the plane path (else 0), *ms = the best milliseconds per step (0 when taken
+ * from the cache); selm_spread_shape reports the spread's shape. */
 int selm_autotune(selm_ctx *ctx, int steps, int shape[2], double *ms);
+
+/* The spread launch's shape the next step runs: out[0] = split threshold (0:
+ * no split list, e.g. for systems of fewer than 2048 beads), out[1] =
+ * thermal-forcing workgroups in the spread launch (0 off the plane path). */
+int selm_spread_shape(selm_ctx *ctx, int out[2]);
 
 /* Passes of the 3-D FFT in the ctx's step: 3 (plane path: XY, Z, XY; stages
  * 4 and 6 are then empty), 5 (transposing X, Y, Z, Y, X) or 0 (runtime-length
@@ -306,6 +317,7 @@ int selm_pool_retire(selm_pool *pool, int slot);
 /* selm_autotune for a pool: the shapes are timed on the pool's current
  * systems stepped together, and the pool's device state is restored. */
 int selm_pool_autotune(selm_pool *pool, int steps, int shape[2], double *ms);
+int selm_pool_spread_shape(selm_pool *pool, int out[2]);
 int selm_pool_stats(selm_pool *pool, size_t *used, size_t *budget, int *active);
 void *selm_pool_stream(selm_pool *pool);
 

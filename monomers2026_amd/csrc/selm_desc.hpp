@@ -41,11 +41,12 @@ struct RepDev {
     int M;
     double *x, *v, *fc, *ftot, *m, *fext, *tf, *sg;
     int *bcnt;                 // per tile: beads inserted this step (may exceed kBucketCap)
-    int *slist;                // [ntiles] split tiles of the step (more than kSplitAt bucket
-                               // entries), in the order their count passed kSplitAt
-    int split;                 // 1: tiles of more than kSplitAt entries are listed in slist and
-                               // taken first by the spread (systems of at least kSplitMinBeads
-                               // beads); 0: every tile in its own slot
+    int *slist;                // [ntiles] split tiles of the step (more than split_at bucket
+                               // entries), in the order their count passed split_at
+    int split_at;              // > 0: tiles of more than split_at entries are listed in slist
+                               // and taken first by the spread (systems of at least
+                               // kSplitMinBeads beads; the batch's spread shape sets the
+                               // threshold); 0: every tile in its own slot
     double *bucket;            // [ntiles][kBucketCap][6]: s[3] and t_f[3] of the beads inserted
                                // this step, in insertion order
     int *ovf;                  // [27*M][2] (tile, bead) pairs beyond a full bucket
@@ -88,7 +89,8 @@ struct RepDev {
 constexpr int kMaxTilesPerBead = 27;
 constexpr int kBucketCap = 256;
 constexpr int kSpreadChunk = 32;  // beads a spread wave-group prepares and adds per round
-constexpr int kSplitAt = 64;      // tiles with more bucket entries are taken first by the spread
+constexpr int kSplitAt = 64;      // default split threshold: tiles with more bucket entries are
+                                  // taken first by the spread
 constexpr int kSplitMinBeads = 2048;
 
 // A replica's ctr[8] (zeroed by init), by step parity: [0], [1] the (tile,

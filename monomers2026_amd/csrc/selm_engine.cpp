@@ -692,6 +692,15 @@ static int env_zq_cols() {
     return (v && std::strcmp(v, "9") == 0) ? 9 : 17;
 }
 
+// a non-negative integer from the environment, else `def`
+static int env_count(const char *name, int def) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return def;
+    char *end = nullptr;
+    const long x = std::strtol(v, &end, 10);
+    return (end && *end == 0 && x >= 0 && x < (1L << 20)) ? (int)x : def;
+}
+
 // ---------------------------------------------------------------------------
 // A batch of replicas stepped together
 // ---------------------------------------------------------------------------
@@ -701,6 +710,8 @@ struct Batch {
     bool force_generic_fft = false;
     bool allow_plane = true;        // three-pass plane FFT path where the shapes allow it
     int zq_cols = 17;               // plane path: Z-pass columns per workgroup (17 or 9)
+    int split_at = kSplitAt;        // spread: tiles of more entries are taken first (0: none)
+    int noise_blocks = 0;           // spread launch (plane path): forcing workgroups (0: default)
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -760,6 +771,9 @@ struct Batch {
         long long *smo = sfz + (nrep + 1), *ssp = smo + (nrep + 1);
         bool any_noise = false;
         const size_t csz = fp64 ? 16 : 8;
+        // a split threshold must be a bucket slot (the bead update lists a
+        // tile when its count passes it)
+        const int sa = split_at > 0 && split_at < kBucketCap ? split_at : 0;
         int twmax = 0;
         for (int r = 0; r < nrep; r++)
             for (int a = 0; a < 3; a++) twmax = std::max(twmax, reps[r]->G.n[a]);
@@ -802,12 +816,12 @@ struct Batch {
             const Geometry &G = R.G;
             h[r] = R.D;
             h[r].M = R.M;
-            h[r].split = R.M >= kSplitMinBeads ? 1 : 0;
+            h[r].split_at = R.M >= kSplitMinBeads ? sa : 0;
             sb[r + 1] = sb[r] + R.M;
             stl[r + 1] = stl[r] + G.ntiles;
             sen[r + 1] = sen[r] + G.N;
             smo[r + 1] = smo[r] + G.Nc;
-            ssp[r + 1] = ssp[r] + (R.M >= kSplitMinBeads ? G.gsplit : 0);
+            ssp[r + 1] = ssp[r] + (R.M >= kSplitMinBeads && sa > 0 ? G.gsplit : 0);
             any_noise = any_noise || R.D.noise;
             if (plane) {
                 long long ixy, iz;
@@ -882,6 +896,8 @@ struct Batch {
         for (int r = 0; r < nrep; r++) L.aux_bonds = L.aux_bonds && reps[r]->ang_a.empty() && reps[r]->lj_nt == 0;
         L.okmax = okmax;
         L.zq_cols = zq_cols;
+        L.split_at = sa;
+        L.noise_blocks = noise_blocks;
         dirty = false;
         return SELM_OK;
     }
@@ -1039,35 +1055,137 @@ using namespace selm;
 // selm_ctx: one system
 // ===========================================================================
 // ---------------------------------------------------------------------------
-// Autotuning of the FFT pass shapes
+// Autotuning of the FFT pass shapes and of the spread launch's shape
 // ---------------------------------------------------------------------------
-// Candidates: the three-pass plane path with 17 or 9 Z-pass columns per
-// workgroup, and the five-pass transposing path.  Each is timed over `steps`
-// graph replays after kGraphSteps + 1 warm-up steps (which instantiate both
-// graphs); the device memory holding the
-// batch's state (the regions in `mem`) is copied aside first and restored
-// after every candidate, so tuning leaves the systems exactly as it found
-// them.  The fastest shape is cached per (precision, grid shapes) for the
-// process; a cached batch shape is applied without timing.
+// First the FFT shape: the three-pass plane path with 17 or 9 Z-pass columns
+// per workgroup, and the five-pass transposing path.  Then, with that FFT
+// shape, the spread's: the split threshold (tiles of more bucket entries are
+// listed by the bead update and taken first by their own workgroups; 32, 64,
+// 128 or no list) and, on the plane path, the number of thermal-forcing
+// workgroups sharing the spread launch (the default for the batch's modes,
+// half or twice it).  The spread sums are order-independent fixed-point
+// sums and each forcing mode is drawn by one thread whatever the grid, so
+// the spread candidates give identical bits; the FFT shapes agree to FP32
+// rounding.
+//
+// Each candidate is timed over `steps` graph replays after kGraphSteps + 1
+// warm-up steps (which instantiate both graphs), in each of kTuneRounds
+// rounds that interleave the candidates, and keeps its fastest round (the
+// shapes differ by a few percent, less than one round's spread).  The
+// device memory holding the batch's state (the regions in `mem`) is copied
+// aside first and restored after every candidate, so tuning leaves the
+// systems exactly as it found them.  The chosen shape is cached per
+// (precision, grid shapes, bead counts) for the process; a cached shape is
+// applied without timing.
 struct TuneRegion {
     void *ptr;
     size_t bytes;
 };
 
+struct TuneShape {
+    int passes, zcols;  // FFT: 3 (plane) with zcols 17 or 9, or 5
+    int split_at;       // spread: split threshold (0: no split list)
+    int noise_blocks;   // spread launch (plane path): forcing workgroups (0: default)
+};
+
 static std::mutex g_tune_mu;
-static std::map<std::string, std::pair<int, int>> g_tune_cache;  // key -> (passes, zcols)
+static std::map<std::string, TuneShape> g_tune_cache;
 
 static std::string tune_key(const Batch &B) {
     std::string k = B.fp64 ? "fp64" : "fp32";
     for (const Replica *r : B.reps)
-        k += ":" + std::to_string(r->G.n[0]) + "x" + std::to_string(r->G.n[1]) + "x" + std::to_string(r->G.n[2]);
+        k += ":" + std::to_string(r->G.n[0]) + "x" + std::to_string(r->G.n[1]) + "x" + std::to_string(r->G.n[2]) +
+             "/" + std::to_string(r->M);
     return k;
 }
 
-static void apply_shape(Batch &B, int passes, int zcols) {
-    B.allow_plane = passes == 3;
-    if (zcols) B.zq_cols = zcols;
+static void apply_shape(Batch &B, const TuneShape &t) {
+    B.allow_plane = t.passes == 3;
+    if (t.zcols) B.zq_cols = t.zcols;
+    B.split_at = t.split_at;
+    B.noise_blocks = t.noise_blocks;
     B.dirty = true;
+}
+
+static void report_shape(const Batch &B, int shape[2]) {
+    shape[0] = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
+    shape[1] = B.L.plane ? B.L.zq_cols : 0;
+}
+
+// Times each available candidate (kTuneRounds interleaved rounds, best round
+// each); returns the index of the fastest in *best and its time in *best_ms.
+// `avail[c]` is decided in the first round from the shape the batch takes.
+static int tune_candidates(Batch &B, const std::vector<TuneRegion> &mem, void *save, int steps,
+                           const std::vector<TuneShape> &cand, bool fft_stage, int *best, double *best_ms,
+                           std::string &err) {
+    constexpr int kTuneRounds = 3;
+    hipStream_t st = B.stream;
+    const int nc = (int)cand.size();
+    std::vector<double> cms(nc, -1.0);
+    std::vector<bool> avail(nc, false);
+    int rc = SELM_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int round = 0; round < kTuneRounds && rc == SELM_OK; round++) {
+        for (int c = 0; c < nc && rc == SELM_OK; c++) {
+            apply_shape(B, cand[c]);
+            rc = B.ensure(err);
+            if (rc) break;
+            if (round == 0) {
+                if (fft_stage) {
+                    // a shape the batch cannot take falls back to another;
+                    // the five-pass candidate (last) is the fallback
+                    const int got = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
+                    bool any = false;
+                    for (int q = 0; q < c; q++) any = any || avail[q];
+                    avail[c] = got == cand[c].passes || (c == nc - 1 && !any);
+                } else {
+                    avail[c] = true;
+                }
+            }
+            if (!avail[c]) continue;
+            // warm-up long enough to capture and instantiate both graphs
+            // (one step and kGraphSteps steps), so the timed region below
+            // holds graph launches only
+            rc = B.step(Batch::kGraphSteps + 1, st, err);
+            if (rc) break;
+            if (B.use_graph && (!B.exec || !B.exec_n)) {
+                err = "autotune: graphs not instantiated before timing";
+                rc = SELM_ERR_STATE;
+                break;
+            }
+            HIPCHK(hipEventRecord(e0, st));
+            rc = B.step(steps, st, err);
+            if (rc) break;
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            const double per = (double)t / steps;
+            if (cms[c] < 0.0 || per < cms[c]) cms[c] = per;
+            size_t o = 0;
+            for (const auto &m : mem) {
+                HIPCHK(hipMemcpyAsync(m.ptr, (char *)save + o, m.bytes, hipMemcpyDeviceToDevice, st));
+                o += m.bytes;
+            }
+            HIPCHK(hipStreamSynchronize(st));
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *best = -1;
+    for (int c = 0; c < nc; c++)
+        if (cms[c] >= 0.0 && (*best < 0 || cms[c] < *best_ms)) {
+            *best = c;
+            *best_ms = cms[c];
+        }
+    if (*best < 0) {
+        err = "autotune: no candidate shape could be timed";
+        return SELM_ERR_STATE;
+    }
+    return SELM_OK;
 }
 
 static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int steps, int shape[2], double *ms,
@@ -1078,11 +1196,10 @@ static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int step
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            apply_shape(B, it->second.first, it->second.second);
+            apply_shape(B, it->second);
             int rc = B.ensure(err);
             if (rc) return rc;
-            shape[0] = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
-            shape[1] = B.L.plane ? B.L.zq_cols : 0;
+            report_shape(B, shape);
             *ms = 0.0;
             return SELM_OK;
         }
@@ -1098,79 +1215,54 @@ static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int step
         off += m.bytes;
     }
     HIPCHK(hipStreamSynchronize(st));
-    // Each candidate runs `steps` steps in each of kTuneRounds rounds, the
-    // rounds interleaving the candidates, and keeps its fastest round: the
-    // shapes differ by a few percent, less than one round's spread.
-    constexpr int kTuneRounds = 3;
-    const int cand[3][2] = {{3, 17}, {3, 9}, {5, 0}};
-    double cand_ms[3] = {-1.0, -1.0, -1.0};
-    bool avail[3] = {false, false, false};
-    int rc = SELM_OK;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    for (int round = 0; round < kTuneRounds && rc == SELM_OK; round++) {
-        for (int c = 0; c < 3 && rc == SELM_OK; c++) {
-            apply_shape(B, cand[c][0], cand[c][1]);
-            rc = B.ensure(err);
-            if (rc) break;
-            if (round == 0) {
-                const int got = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
-                const bool any = avail[0] || avail[1];
-                avail[c] = got == cand[c][0] || (c == 2 && !any);  // the last is the fallback
-            }
-            if (!avail[c]) continue;  // shape not available for this batch
-            // warm-up long enough to capture and instantiate both graphs
-            // (one step and kGraphSteps steps), so the timed region below
-            // holds graph launches only
-            rc = B.step(Batch::kGraphSteps + 1, st, err);
-            if (rc) break;
-            if (!B.exec || !B.exec_n) {
-                err = "autotune: graphs not instantiated before timing";
-                rc = SELM_ERR_STATE;
-                break;
-            }
-            if (rc) break;
-            HIPCHK(hipEventRecord(e0, st));
-            rc = B.step(steps, st, err);
-            if (rc) break;
-            HIPCHK(hipEventRecord(e1, st));
-            HIPCHK(hipEventSynchronize(e1));
-            float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, e0, e1));
-            const double per = (double)t / steps;
-            if (cand_ms[c] < 0.0 || per < cand_ms[c]) cand_ms[c] = per;
-            size_t o = 0;
-            for (const auto &m : mem) {
-                HIPCHK(hipMemcpyAsync(m.ptr, (char *)save + o, m.bytes, hipMemcpyDeviceToDevice, st));
-                o += m.bytes;
-            }
-            HIPCHK(hipStreamSynchronize(st));
-        }
-    }
+    const TuneShape base{3, 17, B.split_at, B.noise_blocks};
+    std::vector<TuneShape> fft = {{3, 17, base.split_at, base.noise_blocks},
+                                  {3, 9, base.split_at, base.noise_blocks},
+                                  {5, 0, base.split_at, base.noise_blocks}};
     int best = -1;
     double best_ms = 0.0;
-    for (int c = 0; c < 3; c++)
-        if (cand_ms[c] >= 0.0 && (best < 0 || cand_ms[c] < best_ms)) {
-            best = c;
-            best_ms = cand_ms[c];
+    int rc = tune_candidates(B, mem, save, steps, fft, true, &best, &best_ms, err);
+    TuneShape chosen = best >= 0 ? fft[best] : base;
+    if (rc == SELM_OK) {
+        // the spread's shape, with the chosen FFT shape
+        apply_shape(B, chosen);
+        rc = B.ensure(err);
+        if (rc == SELM_OK) {
+            const bool plane = B.L.plane && B.L.tot_modes > 0;
+            const int nb0 = plane ? spread_noise_default(B.L.tot_modes) : 0;
+            std::vector<TuneShape> sp;
+            for (int sa : {kSplitAt, 32, 128, 0}) {
+                TuneShape t = chosen;
+                t.split_at = sa;
+                t.noise_blocks = 0;
+                sp.push_back(t);
+            }
+            if (plane) {
+                for (int nb : {nb0 / 2, nb0 * 2}) {
+                    TuneShape t = chosen;
+                    t.split_at = kSplitAt;
+                    t.noise_blocks = std::max(64, nb);
+                    sp.push_back(t);
+                }
+            }
+            int sbest = -1;
+            double sms = 0.0;
+            rc = tune_candidates(B, mem, save, steps, sp, false, &sbest, &sms, err);
+            if (rc == SELM_OK) {
+                chosen = sp[sbest];
+                best_ms = sms;
+            }
         }
-    if (rc == SELM_OK && best < 0) {
-        err = "autotune: no FFT shape could be timed";
-        rc = SELM_ERR_STATE;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     (void)hipFree(save);
     if (rc) return rc;
-    apply_shape(B, cand[best][0], cand[best][1]);
+    apply_shape(B, chosen);
     rc = B.ensure(err);
     if (rc) return rc;
-    shape[0] = B.L.fast ? (B.L.plane ? 3 : 5) : 0;
-    shape[1] = B.L.plane ? B.L.zq_cols : 0;
+    report_shape(B, shape);
     *ms = best_ms;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = {cand[best][0], cand[best][1]};
+    g_tune_cache[key] = chosen;
     return SELM_OK;
 }
 
@@ -1227,6 +1319,8 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     // SELM_ZQ_COLS=9 starts the plane path with 9 Z-pass columns per workgroup
     c->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
     c->batch.zq_cols = env_zq_cols();
+    c->batch.split_at = env_count("SELM_SPLIT_AT", kSplitAt);
+    c->batch.noise_blocks = env_count("SELM_FORCING_WG", 0);
     if (c->batch.init_streams(1, err) != SELM_OK) {
         c->batch.release();
         delete c;
@@ -1461,6 +1555,24 @@ int selm_autotune(selm_ctx *c, int steps, int shape[2], double *ms) {
     return autotune_batch(c->batch, mem, steps, shape, ms, err);
 }
 
+// the spread launch's shape the next step runs: split threshold (0: no
+// split list) and thermal-forcing workgroups (0 off the plane path)
+static int spread_shape_of(Batch &B, int out[2], std::string &err) {
+    int rc = B.ensure(err);
+    if (rc) return rc;
+    out[0] = B.L.tot_split > 0 ? B.L.split_at : 0;
+    out[1] = B.L.plane && B.L.tot_modes > 0
+                 ? (B.L.noise_blocks > 0 ? B.L.noise_blocks : spread_noise_default(B.L.tot_modes))
+                 : 0;
+    return SELM_OK;
+}
+
+int selm_spread_shape(selm_ctx *c, int out[2]) {
+    CTX_CHECK(c);
+    if (!out) return SELM_ERR_ARG;
+    return spread_shape_of(c->batch, out, c->err);
+}
+
 int selm_fft_passes(selm_ctx *c) {
     CTX_CHECK(c);
     int rc = c->batch.ensure(c->err);
@@ -1594,6 +1706,8 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->batch.force_generic_fft = std::getenv("SELM_FFT_GENERIC") != nullptr;
     P->batch.allow_plane = !env_is_zero("SELM_FFT_PLANES");
     P->batch.zq_cols = env_zq_cols();
+    P->batch.split_at = env_count("SELM_SPLIT_AT", kSplitAt);
+    P->batch.noise_blocks = env_count("SELM_FORCING_WG", 0);
     if (P->batch.init_streams(pp->max_replicas, P->err) != SELM_OK) {
         P->batch.release();
         delete P;
@@ -1831,6 +1945,11 @@ int selm_pool_autotune(selm_pool *P, int steps, int shape[2], double *ms) {
     }
     std::vector<TuneRegion> mem{{P->arena, end}};
     return autotune_batch(P->batch, mem, steps, shape, ms, P->err);
+}
+
+int selm_pool_spread_shape(selm_pool *P, int out[2]) {
+    if (!P || !out) return SELM_ERR_ARG;
+    return spread_shape_of(P->batch, out, P->err);
 }
 
 int selm_pool_stats(selm_pool *P, size_t *used, size_t *budget, int *active) {

@@ -191,9 +191,11 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
             e[0] = f64x2{s[0], s[1]};
             e[1] = f64x2{s[2], t0};
             e[2] = f64x2{t1, t2};
-            // a tile of more than kSplitAt entries goes to the split list
-            // (once per step: the count passes kSplitAt once)
-            if (slot == kSplitAt && R.split) gbl(R.slist)[atomicAdd(&R.ctr[kCtrSplits + (step & 1)], 1)] = t;
+            // a tile of more than split_at entries goes to the split list
+            // (once per step: the count passes split_at once; split_at is
+            // below kBucketCap, so the slot is a bucket slot)
+            if (slot == R.split_at && R.split_at > 0)
+                gbl(R.slist)[atomicAdd(&R.ctr[kCtrSplits + (step & 1)], 1)] = t;
         } else {
             const int o = atomicAdd(&R.ctr[step & 1], 1);
             gbl(R.ovf)[2 * o] = t;
@@ -232,7 +234,7 @@ constexpr int kSpreadThreads = 256;
 // of 512, 1024, 2048, 8192 gave 1.148, 1.093, 1.105, 1.112 ms per step.
 constexpr int kSpreadModesPerThread = 8;
 constexpr int kSpreadNoiseMax = 1024;
-static inline int spread_noise_blocks(long long modes) {
+int spread_noise_default(long long modes) {
     if (modes <= 0) return 0;
     long long b = modes / ((long long)kSpreadThreads * kSpreadModesPerThread);
     b = b / 256 * 256;
@@ -533,7 +535,7 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
 // listed: workgroup j of a replica takes split tiles j, j + G, ... (G = its
 // share); they come early so that the fullest tiles start at once.  The
 // remaining workgroups (work table st_tile) take one tile each: every touched
-// tile, or with R.split those of at most kSplitAt bucket entries.
+// tile, or with R.split_at > 0 those of at most split_at bucket entries.
 template <typename T>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restrict__ reps, int nrep,
                                                            const long long *__restrict__ st_modes,
@@ -583,7 +585,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
     }
     const int cnt = R.bcnt[t];
     // untouched tiles keep a stale stamp; split tiles have their own workgroups
-    if (cnt == 0 || (R.split && cnt > kSplitAt)) return;
+    if (cnt == 0 || (R.split_at > 0 && cnt > R.split_at)) return;
     // issue priority over the thermal forcing
     __builtin_amdgcn_s_setprio(3);
     spread_tile<T, 1>(R, t, cnt, step, par, sh, grp);
@@ -1091,7 +1093,8 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
             break;
         case kStageSpread: {
             // the plane path's thermal forcing (FP32-bulk only)
-            const int nnoise = L.plane ? spread_noise_blocks(L.tot_modes) : 0;
+            const int nnoise =
+                L.plane && L.tot_modes > 0 ? (L.noise_blocks > 0 ? L.noise_blocks : spread_noise_default(L.tot_modes)) : 0;
             hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)(nnoise + L.tot_split + L.tot_tile)),
                                dim3(kSpreadThreads), 0, st, reps, nrep, L.st_modes, L.tot_modes, nnoise, L.st_split,
                                L.tot_split, L.st_tile, L.tot_tile);

@@ -39,6 +39,9 @@ struct StepLaunch {
                                      // replica has angle or lj/cut terms); no bond stage of its own
     const long long *st_split;       // work table of the spread's split-tile workgroups
     long long tot_split;
+    int split_at;                    // spread shape: split threshold (0: no split list)
+    int noise_blocks;                // spread shape (plane path): thermal-forcing workgroups of
+                                     // the spread launch (0: the default for the batch's modes)
     const long long *st_modes;       // thermal-forcing work table (half-spectrum modes)
     long long tot_modes;             // 0 when no replica has thermal forcing (or not fast)
     hipStream_t st2;                 // forked stream for the thermal forcing (five-pass path)
@@ -46,6 +49,9 @@ struct StepLaunch {
 };
 
 void launch_step(const StepLaunch &L, hipStream_t st);
+// thermal-forcing workgroups of a plane-path spread launch for `modes` modes
+// when the batch's spread shape leaves the count open
+int spread_noise_default(long long modes);
 void launch_stage(const StepLaunch &L, int stage, hipStream_t st);
 void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st);
 void launch_fill_f64(double *p, long long n, double v, hipStream_t st);

@@ -47,6 +47,7 @@ EXPORTS = [
     "selm_msd", "selm_set_angles", "selm_set_pair_lj", "selm_fft_passes", "selm_autotune",
     "selm_pool_autotune", "selm_status", "selm_pool_status", "selm_check_beads", "selm_msd_noalloc",
     "selm_set_timestep", "selm_pool_set_field", "selm_pool_set_timestep",
+    "selm_spread_shape", "selm_pool_spread_shape",
 ]
 
 
@@ -129,6 +130,8 @@ def load(path=None):
     L.selm_pool_timestep.argtypes = [vp, C.c_int]
     L.selm_pool_timestep.restype = C.c_longlong
     L.selm_set_timestep.argtypes = [vp, C.c_longlong]
+    L.selm_spread_shape.argtypes = [vp, ip]
+    L.selm_pool_spread_shape.argtypes = [vp, ip]
     L.selm_pool_set_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.selm_pool_set_timestep.argtypes = [vp, C.c_int, C.c_longlong]
     L.selm_pool_retire.argtypes = [vp, C.c_int]
@@ -314,12 +317,22 @@ class System:
         return out
 
     def autotune(self, steps=20):
-        """Time the FFT pass shapes on this system and keep the fastest
-        (selm_autotune).  Returns {"passes", "zcols", "ms_per_step"}."""
+        """Time the FFT pass shapes, then the spread launch's shapes, on this
+        system and keep the fastest (selm_autotune).  Returns {"passes",
+        "zcols", "split_at", "forcing_workgroups", "ms_per_step"}."""
         shape = np.zeros(2, np.int32)
         ms = np.zeros(1)
         self._chk(self.L.selm_autotune(self.h, int(steps), _i(shape), _d(ms)), "autotune")
-        return {"passes": int(shape[0]), "zcols": int(shape[1]), "ms_per_step": float(ms[0])}
+        sp = self.spread_shape()
+        return {"passes": int(shape[0]), "zcols": int(shape[1]), "split_at": sp[0],
+                "forcing_workgroups": sp[1], "ms_per_step": float(ms[0])}
+
+    def spread_shape(self):
+        """(split threshold, forcing workgroups of the spread launch) of the
+        next step (selm_spread_shape)."""
+        out = np.zeros(2, np.int32)
+        self._chk(self.L.selm_spread_shape(self.h, _i(out)), "spread_shape")
+        return int(out[0]), int(out[1])
 
     def fft_passes(self):
         """3 (plane path), 5 (transposing path) or 0 (runtime-length kernels)."""
@@ -435,7 +448,14 @@ class Pool:
         shape = np.zeros(2, np.int32)
         ms = np.zeros(1)
         self._chk(self.L.selm_pool_autotune(self.h, int(steps), _i(shape), _d(ms)), "pool_autotune")
-        return {"passes": int(shape[0]), "zcols": int(shape[1]), "ms_per_step": float(ms[0])}
+        sp = self.spread_shape()
+        return {"passes": int(shape[0]), "zcols": int(shape[1]), "split_at": sp[0],
+                "forcing_workgroups": sp[1], "ms_per_step": float(ms[0])}
+
+    def spread_shape(self):
+        out = np.zeros(2, np.int32)
+        self._chk(self.L.selm_pool_spread_shape(self.h, _i(out)), "pool_spread_shape")
+        return int(out[0]), int(out[1])
 
     def stats(self):
         used = C.c_size_t(); budget = C.c_size_t(); active = C.c_int()

@@ -77,3 +77,54 @@ def test_plane_z_shapes_agree(gpu_lib, monkeypatch):
         e = np.abs(a - b).max() / max(np.abs(b).max(), 1.0)
         print("plane z shapes 17 vs 9: rel %.2e" % e, flush=True)
         assert e < 2e-6
+
+
+def test_spread_shapes_give_identical_bits(gpu_lib, monkeypatch):
+    """The spread shapes the autotuner chooses between -- split threshold
+    64, 32 or none, and the default, 128 or 1024 thermal-forcing workgroups
+    in the spread launch -- give bitwise identical steps: the spread sums are
+    order-independent fixed-point sums and each forcing mode is drawn by one
+    thread whatever the grid.  4000 beads on 64^3 (split list active)."""
+    from monomers2026_amd import selm, workloads
+    rng = np.random.default_rng(31)
+    x, ij = workloads.chains(rng, 40, 100, 64)
+    w = workloads.Workload("dense4000_64", 64, x, ij)
+    ref = None
+    for sa, wg in (("64", ""), ("32", ""), ("0", ""), ("64", "128"), ("128", "1024")):
+        monkeypatch.setenv("SELM_SPLIT_AT", sa)
+        monkeypatch.setenv("SELM_FORCING_WG", wg)
+        s = w.build_system(selm.PREC_FP32_BULK, True, seed=9)
+        shape = s.spread_shape()
+        assert shape[0] == int(sa)
+        assert shape[1] == (int(wg) if wg else shape[1]) and shape[1] > 0
+        s.step(10)
+        got = _state(s)
+        assert s.status() == 0
+        s.close()
+        print("spread shape", shape, flush=True)
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b), shape
+    monkeypatch.delenv("SELM_SPLIT_AT", raising=False)
+    monkeypatch.delenv("SELM_FORCING_WG", raising=False)
+
+
+def test_autotune_reports_spread_shape(gpu_lib):
+    from monomers2026_amd import selm, workloads
+    rng = np.random.default_rng(32)
+    x, ij = workloads.chains(rng, 30, 100, 64)
+    w = workloads.Workload("dense3000_64", 64, x, ij)
+    s = w.build_system(selm.PREC_FP32_BULK, True, seed=10)
+    s.step(2)
+    before = _state(s)
+    t = s.autotune(4)
+    print("autotune dense3000_64:", t, flush=True)
+    assert t["split_at"] in (0, 32, 64, 128)
+    assert (t["forcing_workgroups"] > 0) == (t["passes"] == 3)
+    assert s.spread_shape() == (t["split_at"], t["forcing_workgroups"])
+    after = _state(s)
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    s.close()
