@@ -68,6 +68,8 @@ struct RepDev {
     int *ljb_of;               // [M] bin of each bead this step
     int *ljb_cnt;              // [ljb_n] count, then fill cursor; zero between steps
     int *ljb_start;            // [ljb_n + 1] first list entry of each bin
+    int *ljb_part;             // [lj_bin_chunks(ljb_n)] bead count, then first list entry, of
+                               // each chunk of kLjChunk consecutive bins
     int *ljb_list;             // [M] bead indices by bin, ascending within a bin
     // ---- scalars
     double dx, lo, hi, L;      // lo[0], hi[0] (reference wraps and grids with dim 0 only)
@@ -87,6 +89,10 @@ struct RepDev {
 // of the three staggered stencils spans cells floor(s-1/2)-1 .. floor(s)+2,
 // which meets at most 3 tiles (2 unless a ragged tile is narrower than 4).
 constexpr int kMaxTilesPerBead = 27;
+// lj/cut bins are scanned in chunks of kLjChunk consecutive bins (one
+// workgroup each); at most 64 bins per axis gives at most 256 chunks
+constexpr int kLjChunk = 1024;
+__host__ __device__ inline int lj_bin_chunks(int nbins) { return (nbins + kLjChunk - 1) / kLjChunk; }
 constexpr int kBucketCap = 256;
 constexpr int kSpreadChunk = 32;  // beads a spread wave-group prepares and adds per round
 constexpr int kSplitAt = 64;      // default split threshold: tiles with more bucket entries are

@@ -148,6 +148,7 @@ static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, i
     b += al((size_t)M * 4) + al((size_t)nt * nt * 4 * 8);
     b += al(((size_t)M + 1) * 4) + al((size_t)nspec * 4) + al((size_t)nspec * 8);
     b += 2 * al((size_t)M * 4) + al((size_t)nbins * 4) + al(((size_t)nbins + 1) * 4);
+    b += al((size_t)lj_bin_chunks(nbins) * 4);
     return b;
 }
 
@@ -263,6 +264,7 @@ static void carve_beads(Replica &R, char *base) {
     R.D.ljb_list = (int *)take((size_t)M * 4);
     R.D.ljb_cnt = (int *)take((size_t)R.ljb_n * 4);
     R.D.ljb_start = (int *)take(((size_t)R.ljb_n + 1) * 4);
+    R.D.ljb_part = (int *)take((size_t)lj_bin_chunks(R.ljb_n) * 4);
     R.D.ljb_n = R.lj_nt > 0 ? R.ljb_n : 0;
     for (int a = 0; a < 3; a++) {
         R.D.ljb_nb[a] = R.ljb_nb[a];
@@ -503,6 +505,8 @@ static int host_set_angles(Replica &R, int na, const int *ijk, const double *K, 
 // largest cutoff, at most kLjMaxBins; used when every axis has at least 3
 // (so the 27 bins around a bead are distinct) and `bins` is set.
 constexpr int kLjMaxBins = 64;
+static_assert((long long)kLjMaxBins * kLjMaxBins * kLjMaxBins <= 256LL * kLjChunk,
+              "the chunk scan of the lj/cut bins takes one workgroup of 256 chunks");
 static void lj_bin_geometry(Replica &R, double rcmax, bool bins) {
     R.ljb_n = 0;
     for (int a = 0; a < 3; a++) R.ljb_nb[a] = 0;
@@ -890,7 +894,11 @@ struct Batch {
         L.twmax = twmax;
         L.fast = fast;
         L.lj_bins = false;
-        for (int r = 0; r < nrep; r++) L.lj_bins = L.lj_bins || reps[r]->D.ljb_n > 0;
+        L.lj_maxbins = 0;
+        for (int r = 0; r < nrep; r++) {
+            L.lj_bins = L.lj_bins || reps[r]->D.ljb_n > 0;
+            L.lj_maxbins = std::max(L.lj_maxbins, reps[r]->D.ljb_n);
+        }
         L.plane = plane;
         L.aux_bonds = plane;
         for (int r = 0; r < nrep; r++) L.aux_bonds = L.aux_bonds && reps[r]->ang_a.empty() && reps[r]->lj_nt == 0;

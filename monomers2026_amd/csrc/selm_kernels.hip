@@ -20,7 +20,8 @@
 //                     only; the compile-time-length X pass computes u* itself)
 //   X, Y, Z+projection, Y, X passes of the 3-D FFT (selm_fft_kernels.hip for
 //                     the compile-time lengths, below for the others)
-//   [k_lj_bin, k_lj_scan, k_lj_fill, k_lj_sort]  lj/cut neighbour bins
+//   [k_lj_bin, k_lj_chunk_sums/scan/apply, k_lj_fill, k_lj_sort]  lj/cut
+//                     neighbour bins
 //                     (only when a replica has them)
 //   k_bond_forces     thread per bead: harmonic bonds and angles, lj/cut pairs
 //                     and the external force at x^{n+1}; advances each
@@ -858,8 +859,8 @@ __global__ __launch_bounds__(256) void k_fft_z_proj(const RepDev *__restrict__ r
 // ===========================================================================
 // ---------------------------------------------------------------------------
 // lj/cut neighbour bins, rebuilt at every force evaluation (beads move):
-// k_lj_bin counts the beads of each bin, k_lj_scan turns the counts into list
-// offsets and fill cursors, k_lj_fill files each bead, and k_lj_sort orders
+// k_lj_bin counts the beads of each bin, k_lj_chunk_* turn the counts into
+// list offsets and fill cursors, k_lj_fill files each bead, and k_lj_sort orders
 // every bin's list by bead index and clears its count for the next step.
 // The lists are thus the same whatever order the atomics ran in.
 // ---------------------------------------------------------------------------
@@ -877,6 +878,13 @@ __global__ __launch_bounds__(256) void k_lj_bin(const RepDev *__restrict__ reps,
     const RepDev &R = reps[r];
     if (R.ljb_n == 0) return;
     const int b = (int)(g - start[r]);
+    if (!(isfinite(R.x[3 * b]) && isfinite(R.x[3 * b + 1]) && isfinite(R.x[3 * b + 2]))) {
+        // a bead that has left the finite numbers is filed in no bin (and
+        // flagged): it would otherwise share one bin with every such bead
+        R.ljb_of[b] = -1;
+        atomicOr(R.status, kStatusNonFinite);
+        return;
+    }
     const int ix = lj_bin_axis(R.x[3 * b], R.ljb_lo[0], R.Lb[0], R.ljb_nb[0]);
     const int iy = lj_bin_axis(R.x[3 * b + 1], R.ljb_lo[1], R.Lb[1], R.ljb_nb[1]);
     const int iz = lj_bin_axis(R.x[3 * b + 2], R.ljb_lo[2], R.Lb[2], R.ljb_nb[2]);
@@ -885,35 +893,81 @@ __global__ __launch_bounds__(256) void k_lj_bin(const RepDev *__restrict__ reps,
     atomicAdd(&R.ljb_cnt[bin], 1);
 }
 
-// one workgroup per replica: exclusive scan of the counts (each thread a
-// contiguous range of bins, then a scan of the 256 range sums)
-__global__ __launch_bounds__(256) void k_lj_scan(const RepDev *__restrict__ reps, int nrep) {
+// Exclusive scan of the bin counts in three launches, every one a grid over
+// (chunk of kLjChunk bins, replica): the chunks' totals, a scan of those
+// totals (one workgroup per replica, at most 256 chunks), then each chunk's
+// own scan from its offset.  Each thread takes 4 consecutive bins.
+constexpr int kLjPer = kLjChunk / 256;
+
+// exclusive prefix over the 256 threads of a workgroup; *total = the sum
+__device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int q = 0; q < w; q++) base += wsum[q];
+    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return base + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_lj_chunk_sums(const RepDev *__restrict__ reps, int nrep) {
+    const RepDev &R = reps[blockIdx.y];
+    const int n = R.ljb_n, a = (int)blockIdx.x * kLjChunk;
+    if (a >= n) return;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kLjPer; k++) {
+        const int i = a + (int)threadIdx.x * kLjPer + k;
+        v += i < n ? R.ljb_cnt[i] : 0;
+    }
+    int tot = 0;
+    block_exclusive_scan(v, &tot);
+    if (threadIdx.x == 0) R.ljb_part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_lj_chunk_scan(const RepDev *__restrict__ reps, int nrep) {
     const RepDev &R = reps[blockIdx.x];
     const int n = R.ljb_n;
     if (n == 0) return;
-    __shared__ int part[256];
-    const int t = threadIdx.x, per = (n + 255) / 256;
-    const int a = min(n, t * per), e = min(n, a + per);
-    int sum = 0;
-    for (int i = a; i < e; i++) sum += R.ljb_cnt[i];
-    part[t] = sum;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int i = 0; i < 256; i++) {
-            const int v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-        R.ljb_start[n] = acc;
+    const int nc = lj_bin_chunks(n);  // <= 256
+    const int t = threadIdx.x;
+    const int v = t < nc ? R.ljb_part[t] : 0;
+    int tot = 0;
+    const int off = block_exclusive_scan(v, &tot);
+    if (t < nc) R.ljb_part[t] = off;
+    if (t == 0) R.ljb_start[n] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_lj_chunk_apply(const RepDev *__restrict__ reps, int nrep) {
+    const RepDev &R = reps[blockIdx.y];
+    const int n = R.ljb_n, a = (int)blockIdx.x * kLjChunk;
+    if (a >= n) return;
+    int c[kLjPer];
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kLjPer; k++) {
+        const int i = a + (int)threadIdx.x * kLjPer + k;
+        c[k] = i < n ? R.ljb_cnt[i] : 0;
+        v += c[k];
     }
-    __syncthreads();
-    int off = part[t];
-    for (int i = a; i < e; i++) {
-        const int c = R.ljb_cnt[i];
-        R.ljb_start[i] = off;
-        R.ljb_cnt[i] = off;  // fill cursor
-        off += c;
+    int tot = 0;
+    int off = R.ljb_part[blockIdx.x] + block_exclusive_scan(v, &tot);
+#pragma unroll
+    for (int k = 0; k < kLjPer; k++) {
+        const int i = a + (int)threadIdx.x * kLjPer + k;
+        if (i < n) {
+            R.ljb_start[i] = off;
+            R.ljb_cnt[i] = off;  // fill cursor
+        }
+        off += c[k];
     }
 }
 
@@ -925,13 +979,16 @@ __global__ __launch_bounds__(256) void k_lj_fill(const RepDev *__restrict__ reps
     const RepDev &R = reps[r];
     if (R.ljb_n == 0) return;
     const int b = (int)(g - start[r]);
-    R.ljb_list[atomicAdd(&R.ljb_cnt[R.ljb_of[b]], 1)] = b;
+    const int bin = R.ljb_of[b];
+    if (bin >= 0) R.ljb_list[atomicAdd(&R.ljb_cnt[bin], 1)] = b;
 }
 
-// one workgroup per replica, a thread per bin: insertion sort of the bin's list
+// a thread per bin (grid over bins x replicas): insertion sort of the bin's
+// list by bead index, and the count cleared for the next evaluation
 __global__ __launch_bounds__(256) void k_lj_sort(const RepDev *__restrict__ reps, int nrep) {
-    const RepDev &R = reps[blockIdx.x];
-    for (int i = threadIdx.x; i < R.ljb_n; i += blockDim.x) {
+    const RepDev &R = reps[blockIdx.y];
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (i < R.ljb_n) {
         const int a = R.ljb_start[i], e = R.ljb_start[i + 1];
         for (int q = a + 1; q < e; q++) {
             const int v = R.ljb_list[q];
@@ -1035,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ 
         const int ti = R.lj_type[b];
         const int s0 = R.spec_ptr[b], s1 = R.spec_ptr[b + 1];
         const double xb = R.x[3 * b], yb = R.x[3 * b + 1], zb = R.x[3 * b + 2];
-        if (R.ljb_n > 0) {
+        if (R.ljb_n > 0 && R.ljb_of[b] >= 0) {
             const int nbx = R.ljb_nb[0], nby = R.ljb_nb[1], nbz = R.ljb_nb[2];
             const int own = R.ljb_of[b];
             const int ix = own % nbx, iy = (own / nbx) % nby, iz = own / (nbx * nby);
@@ -1053,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_bond_forces(const RepDev *__restrict__ 
                     }
                 }
             }
-        } else {
+        } else if (R.ljb_n == 0) {
             for (int j = 0; j < R.M; j++)
                 if (j != b) lj_pair(R, b, j, ti, s0, s1, xb, yb, zb, f0, f1, f2);
         }
@@ -1188,9 +1245,12 @@ void launch_bond_forces(const StepLaunch &L, int advance, hipStream_t st) {
     if (L.lj_bins && L.tot_bead > 0) {
         const unsigned nb = nblk(L.tot_bead, 256);
         hipLaunchKernelGGL(k_lj_bin, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_bead, L.tot_bead);
-        hipLaunchKernelGGL(k_lj_scan, dim3(L.nrep), dim3(256), 0, st, L.reps, L.nrep);
+        const unsigned nch = (unsigned)lj_bin_chunks(L.lj_maxbins);
+        hipLaunchKernelGGL(k_lj_chunk_sums, dim3(nch, L.nrep), dim3(256), 0, st, L.reps, L.nrep);
+        hipLaunchKernelGGL(k_lj_chunk_scan, dim3(L.nrep), dim3(256), 0, st, L.reps, L.nrep);
+        hipLaunchKernelGGL(k_lj_chunk_apply, dim3(nch, L.nrep), dim3(256), 0, st, L.reps, L.nrep);
         hipLaunchKernelGGL(k_lj_fill, dim3(nb), dim3(256), 0, st, L.reps, L.nrep, L.st_bead, L.tot_bead);
-        hipLaunchKernelGGL(k_lj_sort, dim3(L.nrep), dim3(256), 0, st, L.reps, L.nrep);
+        hipLaunchKernelGGL(k_lj_sort, dim3(nblk(L.lj_maxbins, 256), L.nrep), dim3(256), 0, st, L.reps, L.nrep);
     }
     hipLaunchKernelGGL(k_bond_forces, dim3(L.tot_bead > 0 ? nblk(L.tot_bead, 256) : 1), dim3(256), 0, st,
                        L.reps, L.nrep, L.st_bead, L.tot_bead, advance);

@@ -32,6 +32,7 @@ struct StepLaunch {
     int twmax;                       // LDS twiddle slots (elements) preceding the data
     bool fast;                       // compile-time-length FFT kernels (selm_fft_kernels.hip)
     bool lj_bins;                    // some replica bins its beads for lj/cut
+    int lj_maxbins;                  // the most lj/cut bins of one replica
     bool plane;                      // ... as the three-pass plane path (XYf, Zq, XYi)
     int okmax;                       // plane path: LDS ints for one plane's tile stamps
     int zq_cols;                     // plane path: (kx, ky) columns per Z-pass workgroup (17 or 9)
