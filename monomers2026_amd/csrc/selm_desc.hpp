@@ -40,14 +40,15 @@ struct RepDev {
     // ---- beads
     int M;
     double *x, *v, *fc, *ftot, *m, *fext, *tf, *sg;
-    int *bcnt;                 // per tile: beads inserted this step (may exceed kBucketCap)
+    int *bcnt;                 // per tile: beads inserted this step (may exceed bcap)
     int *slist;                // [ntiles] split tiles of the step (more than split_at bucket
                                // entries), in the order their count passed split_at
     int split_at;              // > 0: tiles of more than split_at entries are listed in slist
                                // and taken first by the spread (systems of at least
                                // kSplitMinBeads beads; the batch's spread shape sets the
                                // threshold); 0: every tile in its own slot
-    double *bucket;            // [ntiles][kBucketCap][6]: s[3] and t_f[3] of the beads inserted
+    int bcap;                  // bucket slots per tile (>= kBucketCap; scaled to the bead density)
+    double *bucket;            // [ntiles][bcap][6]: s[3] and t_f[3] of the beads inserted
                                // this step, in insertion order
     int *ovf;                  // [27*M][2] (tile, bead) pairs beyond a full bucket
     int *ctr;                  // [8]: overflow count per step parity [0,1]
@@ -93,7 +94,7 @@ constexpr int kMaxTilesPerBead = 27;
 // workgroup each); at most 64 bins per axis gives at most 256 chunks
 constexpr int kLjChunk = 1024;
 __host__ __device__ inline int lj_bin_chunks(int nbins) { return (nbins + kLjChunk - 1) / kLjChunk; }
-constexpr int kBucketCap = 256;
+constexpr int kBucketCap = 256;  // the fewest bucket slots per tile
 constexpr int kSpreadChunk = 32;  // beads a spread wave-group prepares and adds per round
 constexpr int kSplitAt = 64;      // default split threshold: tiles with more bucket entries are
                                   // taken first by the spread

@@ -130,15 +130,28 @@ static size_t grid_bytes(const Geometry &G, bool fp64) {
     b += al(8) + al(3 * 8);               // ts, diag
     const size_t tcells = (size_t)G.te[0] * G.te[1] * G.te[2];
     b += al((size_t)G.ntiles * 3 * tcells * s);                   // pool
-    b += 2 * al((size_t)G.ntiles * 4) + al((size_t)G.ntiles * kBucketCap * 48) +
-         al((size_t)G.ntiles * 4) + al(8 * 4);
+    b += 2 * al((size_t)G.ntiles * 4) + al((size_t)G.ntiles * 4) + al(8 * 4);
     return b;
 }
 
 // nadj: bond entries (2 per bond), nang: angle entries (3 per angle),
 // nt: lj/cut atom types, nspec: special-partner entries
-static size_t bead_bytes(int M, int nadj, bool fp64, int nang = 0, int nt = 0, int nspec = 0, int nbins = 0) {
+// Bucket slots per spread tile for M beads over ntiles tiles: four times a
+// generous mean (8 tiles per bead; 3.4 is typical for edge-8 tiles), at
+// least kBucketCap and at most kMaxTileBeads, in multiples of 64.  Dense
+// systems thus keep their tiles' entries in the buckets; the overflow list
+// takes only what a tile holds beyond its bucket.
+static int bucket_cap(int M, int ntiles) {
+    if (ntiles <= 0) return kBucketCap;
+    const double want = 4.0 * 8.0 * (double)M / ntiles;
+    const long long c = ((long long)std::ceil(want) + 63) / 64 * 64;
+    return (int)std::min<long long>(kMaxTileBeads, std::max<long long>(kBucketCap, c));
+}
+
+static size_t bead_bytes(int M, int nadj, bool fp64, int ntiles, int nang = 0, int nt = 0, int nspec = 0,
+                         int nbins = 0) {
     size_t b = 0;
+    b += al((size_t)ntiles * bucket_cap(M, ntiles) * 48);  // spread buckets
     b += 7 * al(3 * (size_t)M * 8);
     b += al((size_t)M * 8);
     b += al((size_t)kMaxTilesPerBead * M * 8);
@@ -179,7 +192,8 @@ struct Replica {
     std::vector<double> spec_w;
     int ljb_nb[3] = {0, 0, 0}, ljb_n = 0;  // lj/cut neighbour bins (0: all pairs)
     size_t bytes_beads() const {
-        return bead_bytes(M, (int)adj_i1.size(), fp64, (int)ang_a.size(), lj_nt, (int)spec_j.size(), ljb_n);
+        return bead_bytes(M, (int)adj_i1.size(), fp64, G.ntiles, (int)ang_a.size(), lj_nt, (int)spec_j.size(),
+                          ljb_n);
     }
     bool inited = false;
 };
@@ -218,7 +232,6 @@ static void carve_grid(Replica &R, char *base) {
     R.D.pool = take((size_t)G.ntiles * 3 * tcells * s);
     R.D.tstamp = (int *)take((size_t)G.ntiles * 4);
     R.D.bcnt = (int *)take((size_t)G.ntiles * 4);
-    R.D.bucket = (double *)take((size_t)G.ntiles * kBucketCap * 48);
     R.D.slist = (int *)take((size_t)G.ntiles * 4);
     R.D.ctr = (int *)take(8 * 4);
 }
@@ -233,6 +246,8 @@ static void carve_beads(Replica &R, char *base) {
         return q;
     };
     R.bead_base = base;
+    R.D.bcap = bucket_cap(M, R.G.ntiles);
+    R.D.bucket = (double *)take((size_t)R.G.ntiles * R.D.bcap * 48);
     R.D.x = (double *)take(3 * (size_t)M * 8);
     R.D.v = (double *)take(3 * (size_t)M * 8);
     R.D.fc = (double *)take(3 * (size_t)M * 8);
@@ -1544,7 +1559,7 @@ int selm_footprint(const selm_params *p, int M, int nb, size_t *bytes) {
     const bool fp64 = p->precision == SELM_PREC_FP64;
     int rc = make_geometry(*p, fp64, G, err);
     if (rc) return rc;
-    *bytes = grid_bytes(G, fp64) + bead_bytes(M, 2 * nb, fp64);
+    *bytes = grid_bytes(G, fp64) + bead_bytes(M, 2 * nb, fp64, G.ntiles);
     return SELM_OK;
 }
 
@@ -1819,7 +1834,7 @@ int selm_pool_admit(selm_pool *P, const selm_params *p, int M, const double *x, 
         delete R;
         return rc;
     }
-    size_t gb = grid_bytes(R->G, fp64), bb = bead_bytes(R->M, (int)R->adj_i1.size(), fp64);
+    size_t gb = grid_bytes(R->G, fp64), bb = bead_bytes(R->M, (int)R->adj_i1.size(), fp64, R->G.ntiles);
     size_t off = 0;
     bool ok = P->alloc.alloc(gb + bb, off);
     if (!ok && P->alloc.size - P->alloc.used() >= al(gb + bb)) {

@@ -187,14 +187,14 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
         const int a = q - qa * cx, qb = qa - c * cy;
         const int t = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
         const int slot = atomicAdd(&R.bcnt[t], 1);
-        if (slot < kBucketCap) {
-            SELM_GLOBAL f64x2 *e = gbl((f64x2 *)R.bucket) + 3 * ((long long)t * kBucketCap + slot);
+        if (slot < R.bcap) {
+            SELM_GLOBAL f64x2 *e = gbl((f64x2 *)R.bucket) + 3 * ((long long)t * R.bcap + slot);
             e[0] = f64x2{s[0], s[1]};
             e[1] = f64x2{s[2], t0};
             e[2] = f64x2{t1, t2};
             // a tile of more than split_at entries goes to the split list
             // (once per step: the count passes split_at once; split_at is
-            // below kBucketCap, so the slot is a bucket slot)
+            // below kBucketCap <= bcap, so the slot is a bucket slot)
             if (slot == R.split_at && R.split_at > 0)
                 gbl(R.slist)[atomicAdd(&R.ctr[kCtrSplits + (step & 1)], 1)] = t;
         } else {
@@ -447,9 +447,9 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
     const int ox = tx * te0, oy = ty * te1, oz = tz * R.te[2];
     const SpreadTile tl{nx, ny, nz, ox, oy, oz, min(te0, nx - ox), min(te1, ny - oy), min(R.te[2], nz - oz), te0, te1};
-    const int nb = min(cnt, kBucketCap);
-    const int novf = cnt > kBucketCap ? min(R.ctr[par], kMaxTilesPerBead * R.M) : 0;
-    const SELM_GLOBAL f64x2 *src = gbl((const f64x2 *)R.bucket) + 3 * (long long)t * kBucketCap;
+    const int nb = min(cnt, R.bcap);
+    const int novf = cnt > R.bcap ? min(R.ctr[par], kMaxTilesPerBead * R.M) : 0;
+    const SELM_GLOBAL f64x2 *src = gbl((const f64x2 *)R.bucket) + 3 * (long long)t * R.bcap;
     const int nch = (nb + kSpreadChunk - 1) / kSpreadChunk;
 
     __syncthreads();  // a previous tile's use of LDS is over

@@ -1,8 +1,10 @@
 """Device status flags and dense tiles, through the C ABI.
 
-- A tile met by more beads than its bucket holds (256) takes the overflow
-  pairs; with 600 beads packed into one 8^3 tile the FP64 result still
-  equals the oracle to 1e-12, and nothing is flagged.
+- 600 beads packed into one 8^3 tile, FP64 against the oracle to 1e-12 with
+  nothing flagged: on a 16^3 grid the tile's bucket (sized to the bead
+  density: 2432 slots for 600 beads over 8 tiles) holds them all; on a 64^3
+  grid it has the minimum 256 slots and the rest go through the overflow
+  pairs.
 - More than 16384 beads in one tile's reach is outside the fixed-point range
   of the spread: the kernel exits normally and selm_sync returns
   SELM_ERR_DEVICE with SELM_STATUS_TILE_OVERFLOW raised.
@@ -25,11 +27,11 @@ def _packed(n, lo, hi, edge, seed):
     return x, v, m
 
 
-@pytest.mark.parametrize("noise", [False, True])
-def test_dense_tile_beyond_bucket_fp64_vs_oracle(gpu_lib, noise):
+@pytest.mark.parametrize("noise,edge", [(False, 16), (True, 16), (True, 64)])
+def test_dense_tile_beyond_bucket_fp64_vs_oracle(gpu_lib, noise, edge):
     import oracle_lib as O
     from monomers2026_amd import selm
-    dims, edge = (16, 16, 16), 16.0
+    dims, edge = (edge,) * 3, float(edge)
     x, v, m = _packed(600, 2.0, 6.0, edge, 3)
     ij = np.array([(i, i + 1) for i in range(599)], np.int32)
     K, r0 = np.full(599, 20.0), np.full(599, 0.5)
