@@ -171,3 +171,41 @@ def test_pool_mixed_fft_path_agrees_to_fp32_rounding(gpu_lib):
     assert ex <= 1e-7 and eu <= 1e-5
     assert pool.status(sa) == 0 and pool.status(sb) == 0
     pool.close()
+
+
+def test_pool_compaction_moves_a_block_by_less_than_its_size(gpu_lib):
+    """[A 40^3][B 64^3][free tail]: retiring A leaves a hole smaller than B
+    in front of it.  Admitting C (48^3), which fits in hole + tail but in
+    neither alone, compacts the arena: B moves down by A's footprint, less
+    than its own, so its block is copied in pieces no longer than the
+    distance moved.  B and C then give the same bits as running alone."""
+    from monomers2026_amd import selm, workloads
+    P = selm.PREC_FP32_BULK
+    rng = np.random.default_rng(22)
+    al = lambda b: (b + 255) // 256 * 256
+    ws = {}
+    for name, L, nch in (("a", 40, 1), ("b", 64, 3), ("c", 48, 2)):
+        x, ij = workloads.chains(rng, nch, 50, L)
+        ws[name] = workloads.Workload(name, L, x, ij)
+    fp = {k: al(selm.footprint(w.params(P, True), w.M, w.ij.shape[0])) for k, w in ws.items()}
+    assert fp["a"] < fp["c"] < fp["b"], fp
+    tail = fp["c"] - fp["a"] // 2  # hole + tail >= C; tail < C; hole < C
+    assert fp["a"] + tail >= fp["c"] and tail < fp["c"]
+    pool = selm.Pool(fp["a"] + fp["b"] + tail, 4, precision=P)
+    sa = _admit(pool, ws["a"], P, 201)
+    sb = _admit(pool, ws["b"], P, 202)
+    assert sa is not None and sb is not None
+    pool.step(3)
+    pool.retire(sa)
+    sc = _admit(pool, ws["c"], P, 203)
+    assert sc is not None, "admission after compaction failed"
+    pool.step(5)
+    for key, sl, n, seed in (("b", sb, 8, 202), ("c", sc, 5, 203)):
+        x, v, _, _ = pool.beads(sl)
+        xa, va, ua = _alone(ws[key], P, n, seed)
+        assert pool.timestep(sl) == n
+        assert np.array_equal(x, xa) and np.array_equal(v, va), key
+        assert np.array_equal(pool.field(sl), ua), key
+        assert pool.status(sl) == 0
+    print("compaction with an overlapping move: bitwise equal to alone", flush=True)
+    pool.close()
