@@ -52,11 +52,12 @@ def test_chain64_32_fp64(gpu_lib, noise):
     w = workloads.chain64_32()
     sysg = w.build_system(selm.PREC_FP64, noise, seed=7)
     o = _oracle(w, noise, 7)
-    for n in (1, 9, 90):
+    # the chain64_32 configuration runs 1000 steps
+    for n in (1, 9, 90, 900):
         sysg.step(n)
         o.step(n)
         _compare(sysg, o, FP64_TOL, "chain64_32 noise=%d step=%d" % (noise, o.timestep))
-    assert sysg.timestep == o.timestep
+    assert sysg.timestep == o.timestep == 1000
 
 
 @pytest.mark.parametrize("noise", [False, True])
@@ -65,6 +66,9 @@ def test_chains1000_64_fp64(gpu_lib, noise):
     w = workloads.chains1000_64()
     sysg = w.build_system(selm.PREC_FP64, noise, seed=9)
     o = _oracle(w, noise, 9)
+    # 20 steps: the two implementations round differently (the GPU spread sums
+    # in fixed point, the oracle in bead order), and on this system the
+    # difference in the bond forces grows to 1.2e-12 by step 100 (noise off)
     for n in (1, 19):
         sysg.step(n)
         o.step(n)
