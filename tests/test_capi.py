@@ -98,3 +98,20 @@ def test_resume_entry_points_reject_null_handles():
     assert L.selm_set_timestep(None, 0) == selm.SELM_ERR_ARG
     assert L.selm_pool_set_timestep(None, 0, 0) == selm.SELM_ERR_ARG
     assert L.selm_pool_set_field(None, 0, 0, None) == selm.SELM_ERR_ARG
+
+
+def test_footprint_follows_bead_density():
+    """The spread buckets hold four times a generous mean of entries per
+    tile (at least 256 slots): a dense system's footprint grows with its bead
+    count beyond what the per-bead arrays alone add (host-only)."""
+    from monomers2026_amd import selm
+    p = selm.make_params((24, 24, 24), (0, 0, 0), (24.0,) * 3, precision=selm.PREC_FP32_BULK)
+    base = selm.footprint(p, 100, 0)          # 27 tiles: 256 slots each
+    sparse = selm.footprint(p, 200, 0)        # still 256 slots
+    dense = selm.footprint(p, 8000, 0)        # 9536 slots per tile
+    per_bead = (sparse - base) / 100.0
+    assert per_bead < 27 * 8 + 400           # bead arrays and the overflow list only
+    buckets = dense - base - per_bead * 7900
+    # 8000 beads over 27 tiles: 4 * 8 * 8000 / 27 -> 9536 slots of 48 bytes
+    # (the per-bead estimate carries the 256-byte alignment of each array)
+    assert abs(buckets - 27 * (9536 - 256) * 48) < 0.01 * buckets, buckets
