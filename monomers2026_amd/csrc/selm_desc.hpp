@@ -95,7 +95,6 @@ constexpr int kMaxTilesPerBead = 27;
 constexpr int kLjChunk = 1024;
 __host__ __device__ inline int lj_bin_chunks(int nbins) { return (nbins + kLjChunk - 1) / kLjChunk; }
 constexpr int kBucketCap = 256;  // the fewest bucket slots per tile
-constexpr int kSpreadChunk = 32;  // beads a spread wave-group prepares and adds per round
 constexpr int kSplitAt = 64;      // default split threshold: tiles with more bucket entries are
                                   // taken first by the spread
 constexpr int kSplitMinBeads = 2048;

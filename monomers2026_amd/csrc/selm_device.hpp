@@ -119,8 +119,11 @@ __device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint32_t stream, 
 // rounding each, and -ln(u1) takes one logf: of u1 itself when u1 < 1/2, and
 // of w = RN(1 - x), x = 1-u1, when u1 >= 1/2, with the first-order term of
 // the rounding of w, ((1-w) - x)/w, added back (1-w and that difference are
-// exact by Sterbenz's lemma).  Neither branch of the choice loses the low
-// bits of the uniform, and no lane evaluates two logarithms.
+// exact by Sterbenz's lemma; the term is below 2^-24 in magnitude, so the
+// hardware reciprocal's error in it is far below FP32 rounding).  Neither
+// branch of the choice loses the low bits of the uniform, and no lane
+// evaluates two logarithms.  The radius takes the hardware square root
+// (v_sqrt_f32, within about 1 ulp).
 __device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint32_t stream, uint32_t step,
                                                 uint32_t index, uint32_t slot, float &z0,
                                                 float &z1) {
@@ -134,11 +137,11 @@ __device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint32_t stream, 
     const float x = fmaf((float)(~c[0]), 0x1p-32f, lo0 * 0x1p-53f);
     const bool small = c[0] < 0x80000000u;  // u1 < 1/2
     const float w = small ? u1 : 1.0f - x;
-    const float corr = small ? 0.0f : ((1.0f - w) - x) / w;
+    const float corr = small ? 0.0f : ((1.0f - w) - x) * __builtin_amdgcn_rcpf(w);
     const float mlog = -(logf(w) + corr);
     // t = (b - 2^52) 2^-53 in [-1/2, 1/2), b = c2 2^21 + (c3 >> 11)
     const float t = fmaf((float)(int32_t)(c[2] ^ 0x80000000u), 0x1p-32f, (float)(c[3] >> 11) * 0x1p-53f);
-    const float rad = sqrtf(2.0f * mlog);
+    const float rad = __builtin_amdgcn_sqrtf(2.0f * mlog);
     float sn, co;
     sincospif(2.0f * t, &sn, &co);
     z0 = rad * co;

@@ -216,13 +216,17 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
 // FP64 exponent of the impulses t_f of the tile's beads (so |term| 2^S <=
 // 2^49); integer sums do not depend on the order of the additions.  The
 // tile's beads are therefore taken in whatever order the bucket insertions
-// left them, with no sort, all four waves add concurrently (ds_add_u64), and
-// the result is bitwise the same from run to run and alone or in a batch.
-// The tile is converted to T once when it is written.
+// left them, with no sort, and the result is bitwise the same from run to
+// run and alone or in a batch.  The tile is converted to T once when it is
+// written.
 //
-// Per bead the kernel reads s and t_f (48 bytes) and recomputes the Peskin
-// factors with the bead update's own arithmetic (peskin4 on the same FP64
-// coordinate), so the factors are bitwise those of the gather.
+// One thread per (bead, component) unit: it reads the bead's entry (s and
+// t_f, 48 bytes), computes the 12 Peskin factors in registers with the bead
+// update's own arithmetic (peskin4 on the same FP64 coordinate, so the
+// factors are bitwise those of the gather) and adds the unit's stencil
+// points that lie in the tile with ds_add_u64.  There is no staging and no
+// barrier between units: a tile is one exponent pass, the units, and the
+// write.
 // ===========================================================================
 constexpr int kTileCells = 512;   // tile edge <= 8
 constexpr int kSpreadThreads = 256;
@@ -262,263 +266,139 @@ struct SpreadTile {
     int te0, te1;    // tile edge (cell index strides)
 };
 
-// Per (bead, component) unit u = 3k + d of a staged chunk.  On grids whose
-// edges are all >= tile edge + 3 ("fast"): a = byte offset in the
-// accumulators of the stencil's first cell, mask = the stencil points (bit
-// 16 lk + 4 lj + li) that lie in the tile.  There the offset q = p - o,
-// taken as q - n when q >= e, is the same for the whole stencil, and each
-// lane only adds its l; this gives the cell (p + l) mod n - o of the general
-// form (so n > 8 >= 5):
-// - p in [o, o + e): a wrapped lane (p + l >= n) cannot come back into the
-//   tile, since p + l - n < o + e + 3 - n <= o;
-// - p >= o + e: the stencil meets the tile only through the wrap, where the
-//   cell is p + l - n - o = (q - n) + l;
-// - p < o: a wrapped cell (<= 2) would need o <= 2, while wrapping needs
-//   p >= n - 3, so o > n - 3 > 2.
-// Otherwise ("general"): a, b, mask = the wrapped stencil base p per axis,
-// and each lane wraps its own cell.
-struct SpreadUnit {
-    int a, b;
-    uint64_t mask;
-};
-
-__device__ __forceinline__ unsigned axis_mask4(int q, int e) {
-    unsigned m = 0;
-#pragma unroll
-    for (int l = 0; l < 4; l++) m |= ((unsigned)(q + l) < (unsigned)e ? 1u : 0u) << l;
-    return m;
-}
-
-// v = bit `lane` of the wave-uniform 64-bit mask ? a : b, as one v_cndmask
-// with the mask in an SGPR pair (a plain C++ select of a mask bit compiles
-// to a shift, an and, a compare and the select).
-__device__ __forceinline__ int lane_select(uint64_t mask, int a, int b) {
-    int v;
-    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(v) : "v"(b), "v"(a), "s"(mask));
-    return v;
-}
-
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Unit u = 3k + d of a staged chunk (thread u < 3 nq; entry k holds s[3]
-// and t_f[3]): the 12 Peskin factors w[u][axis][l] of component d at the
-// bead's coordinate s (x factors times -t_f) and the unit's addressing.
+// One (bead, component) unit: bead coordinate sg (grid units) and impulse
+// tf of component d.  The unit's 12 Peskin factors, with the x factors
+// times -tf; per axis the 4 stencil cells, wrapped into [0, n) and taken
+// relative to the tile origin, with a 4-bit mask of those inside the tile;
+// then one fixed-point add per stencil point in the tile, in the order z, y,
+// x with the product (w_z w_y) w_x.  Any grid edge >= 4 works: the 4 cells
+// of an axis are distinct modulo n, so no point is added twice.
 template <typename T>
-__device__ __forceinline__ void spread_prepare(const double *ent, T *w, SpreadUnit *units, int u, int f4,
-                                               const SpreadTile &g, bool fast) {
-    const int k = u / 3, d = u - 3 * k;
-    const double *sg = ent + 6 * k;
-    const double fd = -sg[3 + d];
-    const int n[3] = {g.nx, g.ny, g.nz};
-    int p[3];
+__device__ __forceinline__ void spread_unit(char *accb, const double sg[3], double tf, int d, int f4,
+                                            const SpreadTile &g, double scale) {
+    const int n[3] = {g.nx, g.ny, g.nz}, o[3] = {g.ox, g.oy, g.oz}, e[3] = {g.ex, g.ey, g.ez};
+    const int stride[3] = {8, 8 * g.te0, 8 * g.te0 * g.te1};  // bytes per cell step
+    T w[3][4];
+    int off[3][4];
+    unsigned msk[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         // component d is sampled at its face centre: the other two axes shift by -1/2
         const double r = sg[a] - (d != a ? 0.5 : 0.0);
-        T f[4];
-        const int p0 = peskin4<T>(r, f4, f);
+        const int p = wrap_near(peskin4<T>(r, f4, w[a]), n[a]);
+        unsigned m = 0;
 #pragma unroll
-        for (int l = 0; l < 4; l++) w[12 * u + 4 * a + l] = a == 0 ? f[l] * (T)fd : f[l];  // x carries the force
-        p[a] = wrap_near(p0, n[a]);
+        for (int l = 0; l < 4; l++) {
+            int c = p + l;
+            c -= c >= n[a] ? n[a] : 0;
+            c -= o[a];
+            m |= ((unsigned)c < (unsigned)e[a] ? 1u : 0u) << l;
+            off[a][l] = c * stride[a];
+        }
+        msk[a] = m;
     }
-    SpreadUnit e;
-    if (fast) {
-        int qx = p[0] - g.ox, qy = p[1] - g.oy, qz = p[2] - g.oz;
-        qx -= qx >= g.ex ? g.nx : 0;
-        qy -= qy >= g.ey ? g.ny : 0;
-        qz -= qz >= g.ez ? g.nz : 0;
-        const unsigned mx = axis_mask4(qx, g.ex), my = axis_mask4(qy, g.ey), mz = axis_mask4(qz, g.ez);
-        uint64_t mask = 0;
+    const T fd = (T)(-tf);
 #pragma unroll
-        for (int z = 0; z < 4; z++)
+    for (int l = 0; l < 4; l++) w[0][l] = w[0][l] * fd;  // x carries the force
+    char *base = accb + 8 * d * kTileCells;
 #pragma unroll
-            for (int y = 0; y < 4; y++) {
-                const uint64_t row = ((mz >> z) & (my >> y) & 1u) ? mx : 0u;
-                mask |= row << (16 * z + 4 * y);
-            }
-        e.a = 8 * (d * kTileCells + (qz * g.te1 + qy) * g.te0 + qx);  // bytes
-        e.b = 0;
-        e.mask = mask;
-    } else {
-        e.a = p[0];
-        e.b = p[1];
-        e.mask = (uint64_t)p[2];
-    }
-    units[u] = e;
-}
-
-// Wave wv adds the units u = wv, wv + 4, ... < 3 nq of a prepared chunk, lane
-// = stencil point (li, lj, lk), each lane into a distinct cell or, outside
-// the tile, into its own sink slot that is never read.
-template <typename T>
-__device__ __forceinline__ void spread_add(unsigned long long *acc, const T *w, const SpreadUnit *units, int nq,
-                                           int lane, int wv, const SpreadTile &g, bool fast, double scale) {
-    const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
-    char *accb = (char *)acc;
-    const int sink = 8 * (3 * kTileCells + lane);
-    if (fast) {
-        const int lofs = 8 * (li + g.te0 * (lj + g.te1 * lk));  // bytes
-        // four units per round with all their LDS reads issued first; the
-        // units past 3 nq in the last round are read (inside the buffers)
-        // but add into the sinks
-        constexpr int U = 4;
-        const int nu = __builtin_amdgcn_readfirstlane(3 * nq);
-        for (int u0 = wv; u0 < nu; u0 += 4 * U) {
-            int cm[U];
-            uint64_t m[U];
-            T v[U];
+    for (int lk = 0; lk < 4; lk++) {
+        if (!((msk[2] >> lk) & 1u)) continue;
 #pragma unroll
-            for (int j = 0; j < U; j++) {
-                const int u = min(u0 + 4 * j, 3 * kSpreadChunk - 1);
-                cm[j] = __builtin_amdgcn_readfirstlane(units[u].a);
-                m[j] = uniform64(u0 + 4 * j < nu ? units[u].mask : 0ull);
-                const T *wu = w + 12 * u;
-                v[j] = (wu[8 + lk] * wu[4 + lj]) * wu[li];
-            }
+        for (int lj = 0; lj < 4; lj++) {
+            if (!((msk[1] >> lj) & 1u)) continue;
+            const T wzy = w[2][lk] * w[1][lj];
+            char *row = base + off[2][lk] + off[1][lj];
 #pragma unroll
-            for (int j = 0; j < U; j++) {
-                const int cell = lane_select(m[j], cm[j] + lofs, sink);
-                __hip_atomic_fetch_add((unsigned long long *)(accb + cell), fixed_term((double)v[j], scale),
+            for (int li = 0; li < 4; li++) {
+                if (!((msk[0] >> li) & 1u)) continue;
+                __hip_atomic_fetch_add((unsigned long long *)(row + off[0][li]), fixed_term((double)(wzy * w[0][li]), scale),
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-    } else {
-        for (int u = wv; u < 3 * nq; u += 4) {
-            const int d = u - 3 * (u / 3);
-            const SpreadUnit e = units[u];
-            int cx = e.a + li;
-            cx -= cx >= g.nx ? g.nx : 0;
-            int cy = e.b + lj;
-            cy -= cy >= g.ny ? g.ny : 0;
-            int cz = (int)e.mask + lk;
-            cz -= cz >= g.nz ? g.nz : 0;
-            cx -= g.ox;
-            cy -= g.oy;
-            cz -= g.oz;
-            const bool in =
-                (unsigned)cx < (unsigned)g.ex && (unsigned)cy < (unsigned)g.ey && (unsigned)cz < (unsigned)g.ez;
-            const T *wu = w + 12 * u;
-            const T v = (wu[8 + lk] * wu[4 + lj]) * wu[li];
-            const int cell = in ? 8 * (d * kTileCells + (cz * g.te1 + cy) * g.te0 + cx) : sink;
-            __hip_atomic_fetch_add((unsigned long long *)(accb + cell), fixed_term((double)v, scale),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
     }
 }
 
-// LDS of one wave-group (256 threads) of the spread: a chunk's staged
-// entries, factors and units, and the overflow window's bead list.
-template <typename T>
-struct SpreadGroup {
-    double ent[kSpreadChunk * 6];  // s[3], t_f[3] per bead
-    T w[3 * kSpreadChunk * 12];
-    SpreadUnit units[3 * kSpreadChunk];
-    int list[kSpreadChunk];
-    int nsel;
-    int pad[3];
-};
+constexpr int kSpreadWin = 256;  // overflow pairs a spread workgroup scans per round
 struct SpreadShared {
-    unsigned long long acc[3 * kTileCells + 64];  // cell sums, then one sink slot per lane
-    int emax;
-    int pad[3];
+    unsigned long long acc[3 * kTileCells];  // cell sums
+    int list[kSpreadWin];                    // an overflow window's beads of this tile
+    int nsel, emax;
+    int pad[2];
 };
 
-// Tile t with cnt bucket insertions this step, by NG wave-groups of 256
-// threads: the scale from every impulse of the tile, then bucket chunk
-// ch = NG k + g and overflow window NG k + g by group g in round k, all
-// adding into one LDS accumulator; the tile is rounded to T and written to
-// the pool.  Every thread runs the same sequence of barriers.
-template <typename T, int NG>
+// Tile t with cnt bucket insertions this step: the scale from every impulse
+// of the tile, then the bucket entries' units and the tile's overflow pairs
+// window by window, all adding into one LDS accumulator; the tile is
+// rounded to T and written to the pool.  Every thread runs the same
+// sequence of barriers.
+template <typename T>
 __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uint32_t step, int par,
-                                            SpreadShared &sh, SpreadGroup<T> *grp) {
-    const int tid = threadIdx.x, lt = tid & 255, lane = tid & 63;
-    const int gi = __builtin_amdgcn_readfirstlane(tid >> 8);
-    const int wv = __builtin_amdgcn_readfirstlane(lt >> 6);
-    SpreadGroup<T> &G = grp[gi];
+                                            SpreadShared &sh) {
+    const int tid = threadIdx.x;
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const int te0 = R.te[0], te1 = R.te[1];
     const int tcells = te0 * te1 * R.te[2];
-    // no stencil wraps back into the tile it starts in (see SpreadUnit)
-    const bool fast = nx >= te0 + 3 && ny >= te1 + 3 && nz >= R.te[2] + 3;
     const int tx = t % R.tn[0], ty = (t / R.tn[0]) % R.tn[1], tz = t / (R.tn[0] * R.tn[1]);
     const int ox = tx * te0, oy = ty * te1, oz = tz * R.te[2];
     const SpreadTile tl{nx, ny, nz, ox, oy, oz, min(te0, nx - ox), min(te1, ny - oy), min(R.te[2], nz - oz), te0, te1};
     const int nb = min(cnt, R.bcap);
     const int novf = cnt > R.bcap ? min(R.ctr[par], kMaxTilesPerBead * R.M) : 0;
     const SELM_GLOBAL f64x2 *src = gbl((const f64x2 *)R.bucket) + 3 * (long long)t * R.bcap;
-    const int nch = (nb + kSpreadChunk - 1) / kSpreadChunk;
-
+    const int f4 = R.f4;
+    char *accb = (char *)sh.acc;
     __syncthreads();  // a previous tile's use of LDS is over
     if (tid == 0) sh.emax = 0;
-    for (int q = tid; q < 3 * kTileCells; q += 256 * NG) sh.acc[q] = 0ull;
-    // the scale: every impulse of the tile (bucket entry i's pieces 1 and 2,
-    // and the overflow pairs); the first round's entries load beside them
+    for (int q = tid; q < 3 * kTileCells; q += kSpreadThreads) sh.acc[q] = 0ull;
+    // the scale: the largest exponent of every impulse of the tile (bucket
+    // entries' pieces 1 and 2, and the overflow pairs)
     int e = 0;
-    {
-        const int nq0 = gi < nch ? min(kSpreadChunk, nb - gi * kSpreadChunk) : 0;
-        f64x2 st;
-        if (lt < 3 * nq0) st = src[3 * gi * kSpreadChunk + lt];
-        for (int i = tid; i < nb; i += 256 * NG) {
-            const f64x2 e1 = src[3 * i + 1], e2 = src[3 * i + 2];
-            e = max(e, max(exp_bits(e1.y), max(exp_bits(e2.x), exp_bits(e2.y))));
-        }
-        for (int o = tid; o < novf; o += 256 * NG)
-            if (R.ovf[2 * o] == t) {
-                const int bi = R.ovf[2 * o + 1];
-#pragma unroll
-                for (int d = 0; d < 3; d++) e = max(e, exp_bits(gbl(R.tf)[3 * bi + d]));
-            }
-        if (lt < 3 * nq0) ((f64x2 *)G.ent)[lt] = st;
+    for (int i = tid; i < nb; i += kSpreadThreads) {
+        const f64x2 e1 = src[3 * i + 1], e2 = src[3 * i + 2];
+        e = max(e, max(exp_bits(e1.y), max(exp_bits(e2.x), exp_bits(e2.y))));
     }
+    for (int o = tid; o < novf; o += kSpreadThreads)
+        if (R.ovf[2 * o] == t) {
+            const int bi = R.ovf[2 * o + 1];
+#pragma unroll
+            for (int dd = 0; dd < 3; dd++) e = max(e, exp_bits(gbl(R.tf)[3 * bi + dd]));
+        }
     // the wave's largest exponent by shuffles, then one LDS atomic per wave
     // (an atomic from every lane becomes a per-lane scalar loop)
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) e = max(e, __shfl_xor(e, m, 64));
-    __syncthreads();  // emax = 0 is seen by every thread
-    if (lane == 0) atomicMax(&sh.emax, e);
-
-    for (int k = 0; k * NG < nch; k++) {
-        const int ch = k * NG + gi;
-        const int nq = ch < nch ? min(kSpreadChunk, nb - ch * kSpreadChunk) : 0;
-        if (k > 0) {
-            __syncthreads();  // the previous round is consumed
-            if (lt < 3 * nq) ((f64x2 *)G.ent)[lt] = src[3 * ch * kSpreadChunk + lt];
-        }
-        __syncthreads();
-        if (lt < 3 * nq) spread_prepare<T>(G.ent, G.w, G.units, lt, R.f4, tl, fast);
-        __syncthreads();
-        if (nq > 0) spread_add<T>(sh.acc, G.w, G.units, nq, lane, wv, tl, fast, ldexp(1.0, spread_shift(sh.emax)));
+    __syncthreads();  // emax = 0 and the zeroed cells are seen by every thread
+    if ((tid & 63) == 0) atomicMax(&sh.emax, e);
+    __syncthreads();
+    const int E = sh.emax;
+    const double scale = ldexp(1.0, spread_shift(E));
+    for (int u = tid; u < 3 * nb; u += kSpreadThreads) {
+        const int k = u / 3, dd = u - 3 * k;
+        const f64x2 e0 = src[3 * k], e1 = src[3 * k + 1], e2 = src[3 * k + 2];
+        const double sg[3] = {e0.x, e0.y, e1.x};
+        spread_unit<T>(accb, sg, dd == 0 ? e1.y : (dd == 1 ? e2.x : e2.y), dd, f4, tl, scale);
     }
     // the overflow pairs of this tile, window by window
-    const int nwin = (novf + kSpreadChunk - 1) / kSpreadChunk;
-    for (int k = 0; k * NG < nwin; k++) {
-        const int win = k * NG + gi, o0 = win * kSpreadChunk;
+    for (int o0 = 0; o0 < novf; o0 += kSpreadWin) {
         __syncthreads();
-        if (lt == 0) G.nsel = 0;
+        if (tid == 0) sh.nsel = 0;
         __syncthreads();
-        if (win < nwin && lt < kSpreadChunk && o0 + lt < novf && R.ovf[2 * (o0 + lt)] == t)
-            G.list[atomicAdd(&G.nsel, 1)] = R.ovf[2 * (o0 + lt) + 1];
+        if (o0 + tid < novf && R.ovf[2 * (o0 + tid)] == t) sh.list[atomicAdd(&sh.nsel, 1)] = R.ovf[2 * (o0 + tid) + 1];
         __syncthreads();
-        const int nq = __builtin_amdgcn_readfirstlane(G.nsel);
-        for (int i = lt; i < nq * 6; i += 256) {
-            const int kk = i / 6, q = i - 6 * kk, bi = G.list[kk];
-            G.ent[i] = q < 3 ? gbl(R.sg)[3 * bi + q] : gbl(R.tf)[3 * bi + q - 3];
+        const int nq = __builtin_amdgcn_readfirstlane(sh.nsel);
+        for (int u = tid; u < 3 * nq; u += kSpreadThreads) {
+            const int k = u / 3, dd = u - 3 * k, bi = sh.list[k];
+            const SELM_GLOBAL double *sgb = gbl(R.sg) + 3 * bi;
+            const double sg[3] = {sgb[0], sgb[1], sgb[2]};
+            spread_unit<T>(accb, sg, gbl(R.tf)[3 * bi + dd], dd, f4, tl, scale);
         }
-        __syncthreads();
-        if (lt < 3 * nq) spread_prepare<T>(G.ent, G.w, G.units, lt, R.f4, tl, fast);
-        __syncthreads();
-        if (nq > 0) spread_add<T>(sh.acc, G.w, G.units, nq, lane, wv, tl, fast, ldexp(1.0, spread_shift(sh.emax)));
     }
     __syncthreads();
-    const double inv = ldexp(1.0, -spread_shift(sh.emax));
+    const double inv = ldexp(1.0, -spread_shift(E));
     SELM_GLOBAL T *pool = gbl((T *)R.pool) + (long long)t * 3 * tcells;
-    for (int q = tid; q < 3 * tcells; q += 256 * NG) {
-        const int d = q / tcells;
-        pool[q] = (T)((double)(long long)sh.acc[d * kTileCells + (q - d * tcells)] * inv);
+    for (int q = tid; q < 3 * tcells; q += kSpreadThreads) {
+        const int dd = q / tcells;
+        pool[q] = (T)((double)(long long)sh.acc[dd * kTileCells + (q - dd * tcells)] * inv);
     }
     if (tid == 0) {
         if (cnt > kMaxTileBeads) atomicOr(R.status, kStatusTileOverflow);
@@ -546,7 +426,6 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
                                                            const long long *__restrict__ st_tile,
                                                            long long tot_tile) {
     __shared__ SpreadShared sh;
-    __shared__ SpreadGroup<T> grp[1];
     if constexpr (sizeof(T) == 4) {  // the plane path is FP32-bulk; FP64 launches have nnoise = 0
         if ((int)blockIdx.x < nnoise) {
             for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes;
@@ -568,7 +447,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
         __builtin_amdgcn_s_setprio(3);
         for (int k = j; k < nsplit; k += G) {
             const int t = gbl(R.slist)[k];
-            spread_tile<T, 1>(R, t, R.bcnt[t], step, par, sh, grp);
+            spread_tile<T>(R, t, R.bcnt[t], step, par, sh);
         }
         return;
     }
@@ -589,7 +468,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const RepDev *__restr
     if (cnt == 0 || (R.split_at > 0 && cnt > R.split_at)) return;
     // issue priority over the thermal forcing
     __builtin_amdgcn_s_setprio(3);
-    spread_tile<T, 1>(R, t, cnt, step, par, sh, grp);
+    spread_tile<T>(R, t, cnt, step, par, sh);
 }
 
 // u* over the whole grid into fsp (runtime-length FFT path only)
