@@ -113,34 +113,57 @@ __device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint32_t stream, 
     z1 = rad * s;
 }
 
-// The same pair evaluated in FP32 from the same 53-bit integers, without
-// 64-bit integer arithmetic or branches.  u1 = (a+1) 2^-53 and 1-u1 are both
-// formed from the two 32-bit words with a relative error of at most one FP32
-// rounding each, and -ln(u1) takes one logf: of u1 itself when u1 < 1/2, and
-// of w = RN(1 - x), x = 1-u1, when u1 >= 1/2, with the first-order term of
-// the rounding of w, ((1-w) - x)/w, added back (1-w and that difference are
-// exact by Sterbenz's lemma; the term is below 2^-24 in magnitude, so the
-// hardware reciprocal's error in it is far below FP32 rounding).  Neither
-// branch of the choice loses the low bits of the uniform, and no lane
-// evaluates two logarithms.  The radius takes the hardware square root
-// (v_sqrt_f32, within about 1 ulp).
-__device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint32_t stream, uint32_t step,
-                                                uint32_t index, uint32_t slot, float &z0,
-                                                float &z1) {
-    uint32_t c[4] = {index, step, slot, stream};
+// The six normals of one half-spectrum mode of the thermal forcing (3
+// components x real and imaginary part) from two Philox blocks: block
+// {index, step, 0, stream} gives component 0 from words (0, 1) and
+// component 1 from words (2, 3), block {index, step, 1, stream} gives
+// component 2 from words (0, 1).  Each pair is a Box-Muller transform of
+// two 32-bit uniforms: u1 = (w_a + 1) 2^-32 in (0, 1] and the angle 2 pi t
+// with t = (w_b - 2^31) 2^-32 in [-1/2, 1/2).  (cuRAND's normals, which the
+// reference draws, also come from 32-bit uniforms.)
+__device__ __forceinline__ void mode_blocks(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index,
+                                            uint32_t wa[3], uint32_t wb[3]) {
+    uint32_t c[4] = {index, step, 0u, stream};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    // a = c0 2^21 + (c1 >> 11) in [0, 2^53): u1 = (a + 1) 2^-53
-    const float lo1 = (float)((c[1] >> 11) + 1u);  // exact (< 2^21 + 1)
-    const float u1 = fmaf((float)c[0], 0x1p-32f, lo1 * 0x1p-53f);
-    // 1 - u1 = (2^53 - 1 - a) 2^-53 = ((~c0) 2^21 + (2^21 - 1 - (c1 >> 11))) 2^-53
-    const float lo0 = (float)((0x1FFFFFu - (c[1] >> 11)));  // exact
-    const float x = fmaf((float)(~c[0]), 0x1p-32f, lo0 * 0x1p-53f);
-    const bool small = c[0] < 0x80000000u;  // u1 < 1/2
+    uint32_t e[4] = {index, step, 1u, stream};
+    philox4x32_10(e, (uint32_t)seed, (uint32_t)(seed >> 32));
+    wa[0] = c[0];
+    wb[0] = c[1];
+    wa[1] = c[2];
+    wb[1] = c[3];
+    wa[2] = e[0];
+    wb[2] = e[1];
+}
+
+// Box-Muller pair in FP64 from the 32-bit words (wa, wb): exact uniforms.
+__device__ __forceinline__ void bm32_f64(uint32_t wa, uint32_t wb, double &z0, double &z1) {
+    const double u1 = (double)((uint64_t)wa + 1) * 0x1p-32;
+    const double t = (double)(int32_t)(wb ^ 0x80000000u) * 0x1p-32;
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, co;
+    sincos(2.0 * 3.14159265358979323846 * t, &s, &co);
+    z0 = rad * co;
+    z1 = rad * s;
+}
+
+// The same pair in FP32.  u1 = (wa + 1) 2^-32 and 1 - u1 = (~wa) 2^-32 are
+// both formed with a relative error of at most one FP32 rounding, and
+// -ln(u1) takes one logf: of u1 itself when u1 <= 1/2, and of w = RN(1 - x),
+// x = 1 - u1, when u1 > 1/2, with the first-order term of the rounding of
+// w, ((1 - w) - x)/w, added back (1 - w and that difference are exact by
+// Sterbenz's lemma; the term is below 2^-24 in magnitude, so the hardware
+// reciprocal's error in it is far below FP32 rounding).  Neither branch of
+// the choice loses the low bits of the uniform, and no lane evaluates two
+// logarithms.  The radius takes the hardware square root (v_sqrt_f32,
+// within about 1 ulp).
+__device__ __forceinline__ void bm32_f32(uint32_t wa, uint32_t wb, float &z0, float &z1) {
+    const float u1 = fmaf((float)wa, 0x1p-32f, 0x1p-32f);
+    const float x = (float)(~wa) * 0x1p-32f;
+    const bool small = wa < 0x80000000u;  // u1 <= 1/2
     const float w = small ? u1 : 1.0f - x;
     const float corr = small ? 0.0f : ((1.0f - w) - x) * __builtin_amdgcn_rcpf(w);
     const float mlog = -(logf(w) + corr);
-    // t = (b - 2^52) 2^-53 in [-1/2, 1/2), b = c2 2^21 + (c3 >> 11)
-    const float t = fmaf((float)(int32_t)(c[2] ^ 0x80000000u), 0x1p-32f, (float)(c[3] >> 11) * 0x1p-53f);
+    const float t = (float)(int32_t)(wb ^ 0x80000000u) * 0x1p-32f;
     const float rad = __builtin_amdgcn_sqrtf(2.0f * mlog);
     float sn, co;
     sincospif(2.0f * t, &sn, &co);
@@ -149,21 +172,10 @@ __device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint32_t stream, 
 }
 
 template <typename T>
-struct Normal;
-template <>
-struct Normal<double> {
-    __device__ static void pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t idx,
-                                uint32_t slot, double &a, double &b) {
-        normal_pair_f64(seed, stream, step, idx, slot, a, b);
-    }
-};
-template <>
-struct Normal<float> {
-    __device__ static void pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t idx,
-                                uint32_t slot, float &a, float &b) {
-        normal_pair_f32(seed, stream, step, idx, slot, a, b);
-    }
-};
+__device__ __forceinline__ void bm32(uint32_t wa, uint32_t wb, T &z0, T &z1) {
+    if constexpr (sizeof(T) == 8) bm32_f64(wa, wb, z0, z1);
+    else bm32_f32(wa, wb, z0, z1);
+}
 
 __device__ __forceinline__ int wrap_idx(int p, int n) {
     int q = p % n;

@@ -73,14 +73,17 @@ __device__ __forceinline__ void thermal_mode(const RepDev &R, int i, int j, int 
     const T amp = (T)R.noise_amp;
     const uint32_t nmode = (uint32_t)(((long long)k * ny + j) * xc + i);
     const uint32_t npart = (uint32_t)(((long long)k1 * ny + j1) * xc + i);
+    uint32_t wa[3], wb[3], pa[3], pb[3];
+    mode_blocks(R.seed, kStreamFluid, step, nmode, wa, wb);
+    if (plane) mode_blocks(R.seed, kStreamFluid, step, npart, pa, pb);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         T z0, z1;
-        Normal<T>::pair(R.seed, kStreamFluid, step, nmode, (uint32_t)d, z0, z1);
+        bm32<T>(wa[d], wb[d], z0, z1);
         cplx<T> gk = mk<T>(z0 * re, z1 * im);
         if (plane) {
             T y0, y1;
-            Normal<T>::pair(R.seed, kStreamFluid, step, npart, (uint32_t)d, y0, y1);
+            bm32<T>(pa[d], pb[d], y0, y1);
             gk = mk<T>((gk.x + y0 * re) * (T)0.5, (gk.y - y1 * im) * (T)0.5);
         }
         g[d] = cscale(gk, amp);

@@ -34,6 +34,8 @@
  *   - cuRAND XORWOW / LAMMPS RanMars: replaced by Philox4x32-10 (Salmon et
  *     al., SC'11) with a Box-Muller transform; the normal for a given
  *     (seed, stream, step, index, slot) is a pure function of those numbers.
+ *     The thermal forcing draws the six normals of a mode from two Philox
+ *     blocks (mode_normals); a bead's kick draws one pair (normal_pair).
  *   - LAMMPS bond_style harmonic (E = K (r - r0)^2) and fix addforce, applied
  *     at the call site cuda/SELM_Lagrangian_LAMMPS.cu:122 (setup_minimal).
  */
@@ -97,6 +99,28 @@ void normal_pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index,
 
 constexpr uint32_t kStreamFluid = 0u;
 constexpr uint32_t kStreamBead = 1u;
+
+/* The six normals of one half-spectrum mode (3 components x real and
+ * imaginary part) from two Philox blocks: block {index, step, 0, stream}
+ * gives z[0], z[1] from words (0, 1) and z[2], z[3] from words (2, 3),
+ * block {index, step, 1, stream} gives z[4], z[5] from words (0, 1).  Each
+ * pair is a Box-Muller transform of two 32-bit uniforms: u1 = (w_a + 1)
+ * 2^-32 in (0, 1], the angle 2*pi*t with t = (w_b - 2^31) 2^-32 in
+ * [-1/2, 1/2). */
+void mode_normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index, double z[6]) {
+    uint32_t c[4] = {index, step, 0u, stream}, e[4] = {index, step, 1u, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    philox4x32_10(e, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint32_t wa[3] = {c[0], c[2], e[0]}, wb[3] = {c[1], c[3], e[1]};
+    for (int q = 0; q < 3; q++) {
+        double u1 = (double)((uint64_t)wa[q] + 1) * 0x1p-32;
+        double t = (double)(int32_t)(wb[q] ^ 0x80000000u) * 0x1p-32;
+        double rad = std::sqrt(-2.0 * std::log(u1));
+        double th = 2.0 * kPi * t;
+        z[2 * q] = rad * std::cos(th);
+        z[2 * q + 1] = rad * std::sin(th);
+    }
+}
 
 /* ---------------------------------------------------------------------- */
 /* Peskin 4-point 1-D factor with the reference's branch structure          */
@@ -347,11 +371,9 @@ void fluid_compute_f(Oracle &o, long s) {
                 double re, im;
                 if (k1 != k || j1 != j || i1 != i) { re = coef / std::sqrt(2.0); im = re; }
                 else { re = coef; im = 0.0; }
-                for (int d = 0; d < 3; d++) {
-                    double z0, z1;
-                    normal_pair(o.P.seed, kStreamFluid, (uint32_t)s, (uint32_t)n, (uint32_t)d, z0, z1);
-                    g[d * o.Nc + n] = cd(z0 * re, z1 * im);
-                }
+                double z[6];
+                mode_normals(o.P.seed, kStreamFluid, (uint32_t)s, (uint32_t)n, z);
+                for (int d = 0; d < 3; d++) g[d * o.Nc + n] = cd(z[2 * d] * re, z[2 * d + 1] * im);
             }
     const double scale = std::sqrt(2 * o.P.kB * o.P.T * o.P.dt * o.P.mu / (double)o.N);
     for (int d = 0; d < 3; d++) {
@@ -664,6 +686,9 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
     for (int i = 0; i < 4; i++) out[i] = c[i];
 }
 
+void oracle_mode_normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index, double out[6]) {
+    mode_normals(seed, stream, step, index, out);
+}
 void oracle_normal_pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index, uint32_t slot,
                         double out[2]) {
     normal_pair(seed, stream, step, index, slot, out[0], out[1]);

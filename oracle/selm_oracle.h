@@ -64,6 +64,8 @@ void oracle_fft_c2c_1d(int n, int sign, const double *in_cplx, double *out_cplx)
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void oracle_normal_pair(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index,
                         uint32_t slot, double out[2]);
+/* the six normals of a thermal-forcing mode: component d in out[2d], out[2d+1] */
+void oracle_mode_normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t index, double out[6]);
 /* harmonic bonds + constant external force at the given positions */
 void oracle_bond_forces(void *o, const double *x, double *f);
 void oracle_msd(const float *frames, int nframes, int natoms, const double box[3], int nlags, int mult,

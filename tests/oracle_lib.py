@@ -69,6 +69,7 @@ def lib():
     u32p = C.POINTER(C.c_uint32)
     L.oracle_philox4x32_10.argtypes = [u32p, u32p, u32p]
     L.oracle_normal_pair.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, dp]
+    L.oracle_mode_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, dp]
     L.oracle_bond_forces.argtypes = [vp, dp, dp]
     _lib = L
     return L
@@ -134,6 +135,12 @@ def philox(ctr, key):
 def normal_pair(seed, stream, step, index, slot):
     out = np.zeros(2)
     lib().oracle_normal_pair(seed, stream, step, index, slot, _dp(out))
+    return out
+
+
+def mode_normals(seed, stream, step, index):
+    out = np.zeros(6)
+    lib().oracle_mode_normals(seed, stream, step, index, _dp(out))
     return out
 
 

@@ -169,6 +169,39 @@ def test_normals_moments():
     assert not np.array_equal(O.normal_pair(42, 0, 1, 7, 2), O.normal_pair(42, 1, 1, 7, 2))
 
 
+def test_mode_normals():
+    """The six normals of a thermal-forcing mode: each of the six is a
+    standard normal on its own (moments over 30000 modes), the six are
+    uncorrelated, and the pairs come from the Philox words they are assigned
+    to: with key (0, 0) and counter (0, 0, 0, 0) Random123's known answer is
+    [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8], so component 0's pair
+    has radius sqrt(-2 ln((w0 + 1) 2^-32)) and angle 2 pi (w1 - 2^31) 2^-32,
+    and component 1's the same from (w2, w3)."""
+    z = np.array([O.mode_normals(42, 0, 3, i) for i in range(30000)])
+    for q in range(6):
+        assert abs(z[:, q].mean()) < 0.03, q
+        assert abs(z[:, q].var() - 1.0) < 0.04, q
+        assert abs((z[:, q] ** 4).mean() - 3.0) < 0.25, q
+    c = np.corrcoef(z.T)
+    assert np.abs(c - np.eye(6)).max() < 0.03
+    # tails: P(|z| > 3) = 0.0027 for a standard normal
+    assert abs((np.abs(z) > 3).mean() - 0.0027) < 0.001
+    w = [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    z0 = O.mode_normals(0, 0, 0, 0)
+    for d in range(2):
+        wa, wb = w[2 * d], w[2 * d + 1]
+        rad = np.sqrt(-2.0 * np.log((wa + 1) * 2.0 ** -32))
+        th = 2 * np.pi * (wb - 2 ** 31) * 2.0 ** -32
+        assert abs(z0[2 * d] - rad * np.cos(th)) < 1e-14 and abs(z0[2 * d + 1] - rad * np.sin(th)) < 1e-14, d
+    # the third component comes from the second block (counter slot 1)
+    w1 = O.philox([0, 0, 1, 0], [0, 0])
+    rad = np.sqrt(-2.0 * np.log((w1[0] + 1) * 2.0 ** -32))
+    th = 2 * np.pi * (w1[1] - 2 ** 31) * 2.0 ** -32
+    assert abs(z0[4] - rad * np.cos(th)) < 1e-14 and abs(z0[5] - rad * np.sin(th)) < 1e-14
+    assert np.array_equal(O.mode_normals(42, 0, 1, 7), O.mode_normals(42, 0, 1, 7))
+    assert not np.array_equal(O.mode_normals(42, 0, 1, 7), O.mode_normals(42, 0, 2, 7))
+
+
 # --------------------------------------------------------------------------- whole step
 def _div(u):
     ux, uy, uz = u
