@@ -94,12 +94,8 @@ __device__ __forceinline__ void thermal_mode(const RepDev &R, int i, int j, int 
 // start[r] ..) into nbuf: the row order of C ([kx][ky][kz], kz fastest), or
 // with plane = 1 that of the plane path's P ([kz][kx][ky], ky fastest).
 template <typename T>
-__device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int nrep,
-                                           const long long *__restrict__ start, long long g, int plane) {
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
+__device__ __forceinline__ void noise_mode_of(const RepDev &R, long long m, int plane) {
     if (!R.noise) return;
-    const long long m = g - start[r];
     const int ny = R.n[1], nz = R.n[2];
     const int mi = (int)m;  // Nc < 2^31
     int i, j, k;
@@ -121,6 +117,18 @@ __device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int 
     SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
 #pragma unroll
     for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
+}
+
+// When the wave's active lanes all draw modes of one replica (always for a
+// single system), the replica index is taken from the first active lane so
+// that the descriptor's fields come through scalar loads.
+template <typename T>
+__device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int nrep,
+                                           const long long *__restrict__ start, long long g, int plane) {
+    const int r = find_rep(start, nrep, g);
+    const int r0 = __builtin_amdgcn_readfirstlane(r);
+    if (__ballot(r != r0) == 0) noise_mode_of<T>(reps[r0], g - start[r0], plane);
+    else noise_mode_of<T>(reps[r], g - start[r], plane);
 }
 
 }  // namespace selm

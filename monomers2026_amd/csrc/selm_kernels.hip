@@ -89,20 +89,11 @@ __device__ __forceinline__ T pick4(const T w[4], int l) {
 constexpr int kBeadThreads = 256;
 constexpr int kBeadsPerBlock = kBeadThreads / 16;
 
-// 5 waves per SIMD: 5 workgroups per CU, so 20k beads (1250 workgroups) are
-// resident at once on 256 CUs instead of running in two rounds.
+// Bead b of replica R, by the 16 lanes of its group (active: a real bead;
+// inactive groups mirror the last bead and store nothing).
 template <typename T>
-__global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_bead_update(
-    const RepDev *__restrict__ reps, int nrep,
-                                                              const long long *__restrict__ start,
-                                                              long long total) {
+__device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active) {
     const int sl = threadIdx.x & 15;  // sub-lane: (li, lj) column of the stencil
-    const long long g0 = (long long)blockIdx.x * kBeadsPerBlock + (threadIdx.x >> 4);
-    const bool active = g0 < total;
-    const long long g = active ? g0 : total - 1;  // inactive groups mirror the last bead, store nothing
-    const int r = find_rep(start, nrep, g);
-    const RepDev &R = reps[r];
-    const int b = (int)(g - start[r]);
     const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
     const long long N = R.N;
     const int f4 = R.f4;
@@ -203,6 +194,24 @@ __global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5)
             gbl(R.ovf)[2 * o + 1] = b;
         }
     }
+}
+
+// 5 waves per SIMD: 5 workgroups per CU, so 20k beads (1250 workgroups) are
+// resident at once on 256 CUs instead of running in two rounds.  When every
+// bead of a wave belongs to one replica (always for a single system), the
+// replica index is taken from the first lane, so the descriptor's fields come
+// through scalar loads instead of a chain of vector loads; a wave that
+// straddles two replicas of a batch takes the per-lane form.
+template <typename T>
+__global__ __launch_bounds__(kBeadThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_bead_update(
+    const RepDev *__restrict__ reps, int nrep, const long long *__restrict__ start, long long total) {
+    const long long g0 = (long long)blockIdx.x * kBeadsPerBlock + (threadIdx.x >> 4);
+    const bool active = g0 < total;
+    const long long g = active ? g0 : total - 1;  // inactive groups mirror the last bead, store nothing
+    const int r = find_rep(start, nrep, g);
+    const int r0 = __builtin_amdgcn_readfirstlane(r);
+    if (__ballot(r != r0) == 0) bead_update<T>(reps[r0], (int)(g - start[r0]), active);
+    else bead_update<T>(reps[r], (int)(g - start[r]), active);
 }
 
 // ===========================================================================
