@@ -119,6 +119,49 @@ __device__ __forceinline__ void noise_mode_of(const RepDev &R, long long m, int 
     for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
 }
 
+// Thermal forcing of the modes m0, m0 + stride, ... (at most K, below
+// tot) of replica R into nbuf, plane layout as noise_mode_of.  The
+// wavenumber-table values of all K modes are loaded before the first
+// store: stores count in the same vector memory counter as loads, so a
+// load issued after them would also wait for their completion.
+template <typename T, int K>
+__device__ __forceinline__ void noise_modes_rep(const RepDev &R, long long m0, long long stride, long long tot,
+                                                int plane) {
+    if (!R.noise) return;
+    const int ny = R.n[1], nz = R.n[2];
+    const uint32_t step = (uint32_t)(R.ts[0] + 1);
+    const T *kxt = (const T *)R.ktab[0], *kyt = (const T *)R.ktab[1], *kzt = (const T *)R.ktab[2];
+    int I[K], J[K], Z[K];
+    T lam[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        const long long m = m0 + q * stride;
+        const int mi = (int)(m < tot ? m : m0);  // Nc < 2^31
+        if (plane) {
+            const int ki = div_pos(mi, ny, 1.0 / ny);
+            J[q] = mi - ki * ny;
+            Z[q] = div_pos(ki, R.xc, 1.0 / R.xc);
+            I[q] = ki - Z[q] * R.xc;
+        } else {
+            const int ij = div_pos(mi, nz, 1.0 / nz);
+            Z[q] = mi - ij * nz;
+            I[q] = div_pos(ij, ny, 1.0 / ny);
+            J[q] = ij - I[q] * ny;
+        }
+        lam[q] = (kxt[4 * I[q]] + kyt[4 * J[q]]) + kzt[4 * Z[q]];
+    }
+    SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        const long long m = m0 + q * stride;
+        if (m >= tot) break;
+        cplx<T> gv[3];
+        thermal_mode<T>(R, I[q], J[q], Z[q], lam[q], step, gv);
+#pragma unroll
+        for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
+    }
+}
+
 // When the wave's active lanes all draw modes of one replica (always for a
 // single system), the replica index is taken from the first active lane so
 // that the descriptor's fields come through scalar loads.

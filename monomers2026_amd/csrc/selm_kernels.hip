@@ -99,9 +99,12 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
     const int f4 = R.f4;
     const SELM_GLOBAL double *gx = gbl((const double *)R.x);
 
-    double s[3];
+    double xr[3], s[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) s[d] = (gx[3 * b + d] - R.lo) / R.dx;
+    for (int d = 0; d < 3; d++) xr[d] = gx[3 * b + d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) s[d] = (xr[d] - R.lo) / R.dx;
+    const uint32_t step = (uint32_t)(R.ts[0] + 1);
 
     const int li = sl & 3, lj = sl >> 2;
     const SELM_GLOBAL T *u = gbl((const T *)R.u);
@@ -129,7 +132,13 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
     }
     if (!active) return;
 
-    const uint32_t step = (uint32_t)(R.ts[0] + 1);
+    // The state of component sl (lanes 0..2 of the bead's group) is loaded
+    // before any of the bead's stores.  Stores count in the same vector
+    // memory counter as loads, so a load issued after them would wait for
+    // their completion as well (the compiler cannot move these loads up:
+    // the descriptor's pointers might alias).
+    const int ist = 3 * b + (sl < 3 ? sl : 0);
+    const double vi = gbl(R.v)[ist], fci = gbl(R.fc)[ist], mb = gbl(R.m)[b];
     double tfl = 0.0;  // lanes 0..2 of the bead's group: t_f of component sl
     if (sl < 3) {
         const int d = sl;
@@ -137,7 +146,6 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
         const double sd = d == 0 ? s[0] : (d == 1 ? s[1] : s[2]);
         const int i = 3 * b + d;
         const double dt = R.dt;
-        const double vi = gbl(R.v)[i];
         double tf = -R.ups * (vi - gd) * dt;
         if (R.noise) {
             double z0, z1;
@@ -147,16 +155,16 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
         tfl = tf;
         gbl(R.tf)[i] = tf;
         gbl(R.sg)[i] = sd;
-        const double ft = tf + dt * gbl(R.fc)[i];
+        const double ft = tf + dt * fci;
         gbl(R.ftot)[i] = ft / dt;
         double tmp = vi * dt;
         // a displacement beyond half the box is the reference's sign of a
         // broken run (cuda/SELM_Kernel.cu:363-365, where it is printed)
         int flags = fabs(tmp) > 0.5 * R.L ? kStatusDisplacement : 0;
-        tmp += gx[i];
+        tmp += d == 0 ? xr[0] : (d == 1 ? xr[1] : xr[2]);
         if (tmp > R.hi) tmp -= R.L;
         else if (tmp < R.lo) tmp += R.L;
-        const double vn = vi + ft / gbl(R.m)[b];
+        const double vn = vi + ft / mb;
         flags |= (isfinite(tmp) && isfinite(vn)) ? 0 : kStatusNonFinite;
         if (flags) atomicOr(R.status, flags);
         gbl(R.x)[i] = tmp;
@@ -171,13 +179,27 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
     const int cx = axis_tiles(s[0], nx, R.te[0], tx);
     const int cy = axis_tiles(s[1], ny, R.te[1], ty);
     const int cz = axis_tiles(s[2], nz, R.te[2], tz);
-    const int ncomb = cx * cy * cz;
+    const int ncomb = cx * cy * cz;  // at most 27: each lane takes at most 2 insertions
     const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
-    for (int q = sl; q < ncomb; q += 16) {
-        const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
-        const int a = q - qa * cx, qb = qa - c * cy;
-        const int t = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
-        const int slot = atomicAdd(&R.bcnt[t], 1);
+    // both of a lane's slot claims are issued before any entry is stored, so
+    // neither waits on the other's stores
+    int tq[2], sq[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int q = sl + 16 * h;
+        tq[h] = 0;
+        sq[h] = 0;
+        if (q < ncomb) {
+            const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
+            const int a = q - qa * cx, qb = qa - c * cy;
+            tq[h] = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
+            sq[h] = __hip_atomic_fetch_add(gbl(R.bcnt) + tq[h], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if (sl + 16 * h >= ncomb) break;
+        const int t = tq[h], slot = sq[h];
         if (slot < R.bcap) {
             SELM_GLOBAL f64x2 *e = gbl((f64x2 *)R.bucket) + 3 * ((long long)t * R.bcap + slot);
             e[0] = f64x2{s[0], s[1]};
@@ -440,9 +462,16 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
     __shared__ SpreadShared sh;
     if constexpr (sizeof(T) == 4) {  // the plane path is FP32-bulk; FP64 launches have nnoise = 0
         if ((int)blockIdx.x < nnoise) {
-            for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes;
-                 m += (long long)nnoise * kSpreadThreads)
-                noise_mode<T>(reps, nrep, st_modes, m, 1);
+            const long long stride = (long long)nnoise * kSpreadThreads;
+            if (nrep == 1) {
+                // one system: four modes per call, their table values loaded
+                // before the first store
+                for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes; m += 4 * stride)
+                    noise_modes_rep<T, 4>(reps[0], m, stride, tot_modes, 1);
+            } else {
+                for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes; m += stride)
+                    noise_mode<T>(reps, nrep, st_modes, m, 1);
+            }
             return;
         }
     }
