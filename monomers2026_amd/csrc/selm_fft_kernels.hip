@@ -662,7 +662,7 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
         } else {
             o = a[ky * S + kx];
         }
-        stc(out + idx, o);
+        stc_dev(out + idx, o);
     }
 }
 
@@ -768,7 +768,7 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
     for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
         const int q = idx % Q, dz = idx / Q;
         const int d = dz / N, e = dz - d * N;
-        if (q < cv) stc(Ps + (long long)dz * pl + q, ccast<T>(a[e * S + d * Q + q]));
+        if (q < cv) stc_dev(Ps + (long long)dz * pl + q, ccast<T>(a[e * S + d * Q + q]));
     }
 }
 
@@ -868,7 +868,7 @@ __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const 
     for (int idx = threadIdx.x; idx < NN; idx += NT) {
         const int y = idx / N, x = idx - y * N;
         const cplx<L> v = a[x * S + (y >> 1)];
-        out[idx] = (T)((y & 1) ? v.y : v.x);
+        st_dev(gbl(out + idx), (T)((y & 1) ? v.y : v.x));
     }
 }
 

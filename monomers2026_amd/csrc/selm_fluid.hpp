@@ -116,7 +116,14 @@ __device__ __forceinline__ void noise_mode_of(const RepDev &R, long long m, int 
     thermal_mode<T>(R, i, j, k, lam, (uint32_t)(R.ts[0] + 1), gv);
     SELM_GLOBAL cplx<T> *nb = gbl((cplx<T> *)R.nbuf);
 #pragma unroll
-    for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
+    for (int d = 0; d < 3; d++) {
+        if constexpr (sizeof(T) == 4) {
+            if (plane) stc_dev(nb + d * R.Nc + m, gv[d]);  // the plane path's forcing
+            else stc(nb + d * R.Nc + m, gv[d]);
+        } else {
+            stc(nb + d * R.Nc + m, gv[d]);
+        }
+    }
 }
 
 // Thermal forcing of the modes m0, m0 + stride, ... (at most K, below
@@ -158,7 +165,14 @@ __device__ __forceinline__ void noise_modes_rep(const RepDev &R, long long m0, l
         cplx<T> gv[3];
         thermal_mode<T>(R, I[q], J[q], Z[q], lam[q], step, gv);
 #pragma unroll
-        for (int d = 0; d < 3; d++) stc(nb + d * R.Nc + m, gv[d]);
+        for (int d = 0; d < 3; d++) {
+            if constexpr (sizeof(T) == 4) {
+                if (plane) stc_dev(nb + d * R.Nc + m, gv[d]);  // the plane path's forcing
+                else stc(nb + d * R.Nc + m, gv[d]);
+            } else {
+                stc(nb + d * R.Nc + m, gv[d]);
+            }
+        }
     }
 }
 

@@ -429,7 +429,9 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     SELM_GLOBAL T *pool = gbl((T *)R.pool) + (long long)t * 3 * tcells;
     for (int q = tid; q < 3 * tcells; q += kSpreadThreads) {
         const int dd = q / tcells;
-        pool[q] = (T)((double)(long long)sh.acc[dd * kTileCells + (q - dd * tcells)] * inv);
+        const T v = (T)((double)(long long)sh.acc[dd * kTileCells + (q - dd * tcells)] * inv);
+        if constexpr (sizeof(T) == 4) st_dev(pool + q, v);  // FP32: device-scope, as the plane path's outputs
+        else pool[q] = v;
     }
     if (tid == 0) {
         if (cnt > kMaxTileBeads) atomicOr(R.status, kStatusTileOverflow);
