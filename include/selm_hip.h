@@ -212,8 +212,9 @@ int selm_stage_times(selm_ctx *ctx, int nsteps, double *ms, int nstages);
  * keeps the fastest; then, with that shape, each spread shape: split
  * threshold 32, 64, 128 or none (tiles with more bucket entries are taken
  * first by workgroups of their own) and, on the plane path, half, the
- * default or twice the thermal-forcing workgroups of the spread launch.  The
- * system's device state is restored after each candidate.  The choice is
+ * default or twice the thermal-forcing workgroups of the spread launch; then,
+ * on the plane path, the store shape (bulk outputs stored at device scope,
+ * written through the L2, or plainly).  The system's device state is restored after each candidate.  The choice is
  * cached per (precision, grid shapes, bead counts) for the process, and a
  * cached choice is applied without timing.  shape[0] = FFT passes (3, 5; 0
  * for the runtime-length kernels), shape[1] = Z-pass columns per workgroup on

@@ -731,6 +731,7 @@ struct Batch {
     int zq_cols = 17;               // plane path: Z-pass columns per workgroup (17 or 9)
     int split_at = kSplitAt;        // spread: tiles of more entries are taken first (0: none)
     int noise_blocks = 0;           // spread launch (plane path): forcing workgroups (0: default)
+    int wt_stores = 1;              // plane path: bulk outputs stored at device scope (1) or plainly (0)
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // forked thermal-forcing branch of each step
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -915,6 +916,7 @@ struct Batch {
             L.lj_maxbins = std::max(L.lj_maxbins, reps[r]->D.ljb_n);
         }
         L.plane = plane;
+        L.wt = plane && wt_stores;
         L.aux_bonds = plane;
         for (int r = 0; r < nrep; r++) L.aux_bonds = L.aux_bonds && reps[r]->ang_a.empty() && reps[r]->lj_nt == 0;
         L.okmax = okmax;
@@ -1109,6 +1111,7 @@ struct TuneShape {
     int passes, zcols;  // FFT: 3 (plane) with zcols 17 or 9, or 5
     int split_at;       // spread: split threshold (0: no split list)
     int noise_blocks;   // spread launch (plane path): forcing workgroups (0: default)
+    int wt;             // plane path: bulk outputs at device scope (1) or plain (0)
 };
 
 static std::mutex g_tune_mu;
@@ -1127,6 +1130,7 @@ static void apply_shape(Batch &B, const TuneShape &t) {
     if (t.zcols) B.zq_cols = t.zcols;
     B.split_at = t.split_at;
     B.noise_blocks = t.noise_blocks;
+    B.wt_stores = t.wt;
     B.dirty = true;
 }
 
@@ -1238,10 +1242,10 @@ static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int step
         off += m.bytes;
     }
     HIPCHK(hipStreamSynchronize(st));
-    const TuneShape base{3, 17, B.split_at, B.noise_blocks};
-    std::vector<TuneShape> fft = {{3, 17, base.split_at, base.noise_blocks},
-                                  {3, 9, base.split_at, base.noise_blocks},
-                                  {5, 0, base.split_at, base.noise_blocks}};
+    const TuneShape base{3, 17, B.split_at, B.noise_blocks, B.wt_stores};
+    std::vector<TuneShape> fft = {{3, 17, base.split_at, base.noise_blocks, base.wt},
+                                  {3, 9, base.split_at, base.noise_blocks, base.wt},
+                                  {5, 0, base.split_at, base.noise_blocks, base.wt}};
     int best = -1;
     double best_ms = 0.0;
     int rc = tune_candidates(B, mem, save, steps, fft, true, &best, &best_ms, err);
@@ -1274,6 +1278,21 @@ static int autotune_batch(Batch &B, const std::vector<TuneRegion> &mem, int step
             if (rc == SELM_OK) {
                 chosen = sp[sbest];
                 best_ms = sms;
+            }
+            // then the plane path's store shape: device-scope stores shorten
+            // the gaps between launches, but cost write traffic where the
+            // Z pass's column runs are short (small grids)
+            if (rc == SELM_OK && B.L.plane) {
+                std::vector<TuneShape> ws = {chosen, chosen};
+                ws[0].wt = 1;
+                ws[1].wt = 0;
+                int wbest = -1;
+                double wms = 0.0;
+                rc = tune_candidates(B, mem, save, steps, ws, false, &wbest, &wms, err);
+                if (rc == SELM_OK) {
+                    chosen = ws[wbest];
+                    best_ms = wms;
+                }
             }
         }
     }
@@ -1344,6 +1363,7 @@ int selm_create(const selm_params *p, selm_ctx **out) {
     c->batch.zq_cols = env_zq_cols();
     c->batch.split_at = env_count("SELM_SPLIT_AT", kSplitAt);
     c->batch.noise_blocks = env_count("SELM_FORCING_WG", 0);
+    c->batch.wt_stores = env_count("SELM_WT_STORES", 1) ? 1 : 0;
     if (c->batch.init_streams(1, err) != SELM_OK) {
         c->batch.release();
         delete c;
@@ -1731,6 +1751,7 @@ int selm_pool_create(const selm_pool_params *pp, selm_pool **out) {
     P->batch.zq_cols = env_zq_cols();
     P->batch.split_at = env_count("SELM_SPLIT_AT", kSplitAt);
     P->batch.noise_blocks = env_count("SELM_FORCING_WG", 0);
+    P->batch.wt_stores = env_count("SELM_WT_STORES", 1) ? 1 : 0;
     if (P->batch.init_streams(pp->max_replicas, P->err) != SELM_OK) {
         P->batch.release();
         delete P;

@@ -456,7 +456,7 @@ __device__ __forceinline__ void lds_remap(const LD &ld, const ST &st) {
     __syncthreads();
 }
 
-template <typename T, int N, int NT>
+template <typename T, int N, int NT, bool WT>
 __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const cplx<C> *twx,
                                          const cplx<C> *twy, int *okf, int2 *tab, cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
@@ -662,7 +662,8 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
         } else {
             o = a[ky * S + kx];
         }
-        stc_dev(out + idx, o);
+        if constexpr (WT) stc_dev(out + idx, o);
+        else stc(out + idx, o);
     }
 }
 
@@ -685,7 +686,7 @@ struct ZqShape<9> {
     static constexpr int NT = 256;
 };
 
-template <typename T, int N, int Q>
+template <typename T, int N, int Q, bool WT>
 __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const cplx<C> *tw,
                                         cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
@@ -768,11 +769,14 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
     for (int idx = threadIdx.x; idx < 3 * Q * N; idx += NT) {
         const int q = idx % Q, dz = idx / Q;
         const int d = dz / N, e = dz - d * N;
-        if (q < cv) stc_dev(Ps + (long long)dz * pl + q, ccast<T>(a[e * S + d * Q + q]));
+        if (q < cv) {
+            if constexpr (WT) stc_dev(Ps + (long long)dz * pl + q, ccast<T>(a[e * S + d * Q + q]));
+            else stc(Ps + (long long)dz * pl + q, ccast<T>(a[e * S + d * Q + q]));
+        }
     }
 }
 
-template <typename T, int N, int NT>
+template <typename T, int N, int NT, bool WT>
 __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const cplx<C> *twx,
                                          const cplx<C> *twy, cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
@@ -868,7 +872,8 @@ __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const 
     for (int idx = threadIdx.x; idx < NN; idx += NT) {
         const int y = idx / N, x = idx - y * N;
         const cplx<L> v = a[x * S + (y >> 1)];
-        st_dev(gbl(out + idx), (T)((y & 1) ? v.y : v.x));
+        if constexpr (WT) st_dev(gbl(out + idx), (T)((y & 1) ? v.y : v.x));
+        else out[idx] = (T)((y & 1) ? v.y : v.x);
     }
 }
 
@@ -895,7 +900,7 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
 // only the lengths up to NMAX.  The register budget is then set by the
 // largest length the batch can hold, not by 256.
 #define SELM_FFT_KERNEL(kname, axis, BODY, NTH, ...)                                                  \
-    template <typename T, int NMAX>                                                                   \
+    template <typename T, int NMAX, bool WT = false>                                                  \
     __global__ __launch_bounds__(FastCfg<T>::NTH) __VA_ARGS__ void kname(const RepDev *__restrict__ reps, int nrep, \
                                                             const long long *__restrict__ start,      \
                                                             long long total, int twmax) {             \
@@ -926,9 +931,9 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
     case n: if constexpr (n <= NMAX) xi_body<T, n>(R, item, tw, a); break;
 
 #define SELM_CASE_ZQ(n) \
-    case n: if constexpr (n <= NMAX) zq_body<T, n, 17>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) zq_body<T, n, 17, WT>(R, item, tw, a); break;
 #define SELM_CASE_ZQ9(n) \
-    case n: if constexpr (n <= NMAX) zq_body<T, n, 9>(R, item, tw, a); break;
+    case n: if constexpr (n <= NMAX) zq_body<T, n, 9, WT>(R, item, tw, a); break;
 
 SELM_FFT_KERNEL(k_xf, 0, SELM_CASE_XF, NT)
 // 4 waves per SIMD: two 512-thread workgroups per CU
@@ -940,7 +945,7 @@ SELM_FFT_KERNEL(k_zq9, 2, SELM_CASE_ZQ9, NTZ9, __attribute__((amdgpu_waves_per_e
 // Auxiliary workgroups (plane_block_role) stride over aux_total work items
 // (AUX(i) for each) with aux * NT threads.
 #define SELM_PLANE_KERNEL(kname, BODY, AUX)                                                           \
-    template <typename T, int NMAX>                                                                   \
+    template <typename T, int NMAX, bool WT>                                                          \
     __global__ __launch_bounds__(PlaneNT<NMAX>::v) __attribute__((amdgpu_waves_per_eu(4)))           \
     void kname(const RepDev *__restrict__ reps, int nrep, const long long *__restrict__ start,        \
                long long total, int twmax, int okmax, const long long *__restrict__ aux_start,        \
@@ -971,9 +976,9 @@ SELM_FFT_KERNEL(k_zq9, 2, SELM_CASE_ZQ9, NTZ9, __attribute__((amdgpu_waves_per_e
         }                                                                                             \
     }
 #define SELM_CASE_XYF(n) \
-    case n: if constexpr (n <= NMAX) xyf_body<T, n, PlaneNT<NMAX>::v>(R, item, twx, twy, okf, tab, a); break;
+    case n: if constexpr (n <= NMAX) xyf_body<T, n, PlaneNT<NMAX>::v, WT>(R, item, twx, twy, okf, tab, a); break;
 #define SELM_CASE_XYI(n) \
-    case n: if constexpr (n <= NMAX) xyi_body<T, n, PlaneNT<NMAX>::v>(R, item, twx, twy, a); break;
+    case n: if constexpr (n <= NMAX) xyi_body<T, n, PlaneNT<NMAX>::v, WT>(R, item, twx, twy, a); break;
 #define SELM_AUX_NONE(i) (void)(i)
 #define SELM_AUX_BONDS(i) bond_only_forces(reps, nrep, aux_start, i)
 SELM_PLANE_KERNEL(k_xyf, SELM_CASE_XYF, SELM_AUX_NONE)
@@ -1062,18 +1067,25 @@ void fft_plane_items(bool fp64, const int n[3], int zcols, long long *ixy, long 
 
 bool fft_plane_zcols_ok(int zcols) { return zcols == 17 || zcols == 9; }
 
-// Allow the XY kernels of the size class their dynamic LDS (above 64 KiB).
+// Allow the XY kernels of the size class their dynamic LDS (above 64 KiB),
+// both store shapes.
+template <typename T, int NMAX, bool WT>
+static hipError_t plane_attr_n(size_t lds) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_xyf<T, NMAX, WT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_xyi<T, NMAX, WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return e;
+}
 template <typename T>
 static hipError_t plane_attr_t(int nmax, size_t lds) {
     hipError_t e;
     if (nmax <= 64) {
-        e = hipFuncSetAttribute((const void *)k_xyf<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_xyi<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        e = plane_attr_n<T, 64, true>(lds);
+        if (e == hipSuccess) e = plane_attr_n<T, 64, false>(lds);
     } else {
-        e = hipFuncSetAttribute((const void *)k_xyf<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_xyi<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        e = plane_attr_n<T, 128, true>(lds);
+        if (e == hipSuccess) e = plane_attr_n<T, 128, false>(lds);
     }
     return e;
 }
@@ -1082,26 +1094,26 @@ hipError_t fft_plane_prepare(bool fp64, int twmax, size_t lds) {
     return fp64 ? hipErrorNotSupported : plane_attr_t<float>(twmax, lds);
 }
 
-template <typename T, int NMAX>
+template <typename T, int NMAX, bool WT>
 static void launch_plane_n(const StepLaunch &L, int stage, hipStream_t st) {
     constexpr int NT = PlaneNT<NMAX>::v;
     switch (stage) {
         case kStageFftX:
-            hipLaunchKernelGGL((k_xyf<T, NMAX>), dim3(plane_grid(L.tot_fx, 0)), dim3(NT), L.lds_fx, st, L.reps,
+            hipLaunchKernelGGL((k_xyf<T, NMAX, WT>), dim3(plane_grid(L.tot_fx, 0)), dim3(NT), L.lds_fx, st, L.reps,
                                L.nrep, L.st_fx, L.tot_fx, L.twmax, L.okmax, nullptr, 0LL, 0);
             break;
         case kStageFftZ:
             if (L.zq_cols == 9)
-                hipLaunchKernelGGL((k_zq9<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ9), L.lds_fz, st,
+                hipLaunchKernelGGL((k_zq9<T, NMAX, WT>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ9), L.lds_fz, st,
                                    L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
             else
-                hipLaunchKernelGGL((k_zq<T, NMAX>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
+                hipLaunchKernelGGL((k_zq<T, NMAX, WT>), dim3(pad8(L.tot_fz)), dim3(FastCfg<T>::NTZ), L.lds_fz, st,
                                    L.reps, L.nrep, L.st_fz, L.tot_fz, L.twmax);
             break;
         case kStageFftXi: {
             const long long nb = L.aux_bonds ? L.tot_bead : 0;
             const int aux = aux_blocks(nb, NT, 1, 1 << 20);
-            hipLaunchKernelGGL((k_xyi<T, NMAX>), dim3(plane_grid(L.tot_fx, aux)), dim3(NT), L.lds_fx, st, L.reps,
+            hipLaunchKernelGGL((k_xyi<T, NMAX, WT>), dim3(plane_grid(L.tot_fx, aux)), dim3(NT), L.lds_fx, st, L.reps,
                                L.nrep, L.st_fx, L.tot_fx, L.twmax, L.okmax, L.st_bead, nb, aux);
             break;
         }
@@ -1111,8 +1123,13 @@ static void launch_plane_n(const StepLaunch &L, int stage, hipStream_t st) {
 
 template <typename T>
 static void launch_plane_t(const StepLaunch &L, int stage, hipStream_t st) {
-    if (L.twmax <= 64) launch_plane_n<T, 64>(L, stage, st);
-    else launch_plane_n<T, 128>(L, stage, st);
+    if (L.twmax <= 64) {
+        if (L.wt) launch_plane_n<T, 64, true>(L, stage, st);
+        else launch_plane_n<T, 64, false>(L, stage, st);
+    } else {
+        if (L.wt) launch_plane_n<T, 128, true>(L, stage, st);
+        else launch_plane_n<T, 128, false>(L, stage, st);
+    }
 }
 
 template <typename T, int NMAX>

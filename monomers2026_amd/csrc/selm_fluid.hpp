@@ -93,7 +93,8 @@ __device__ __forceinline__ void thermal_mode(const RepDev &R, int i, int j, int 
 // Thermal forcing of mode g of a batch's work table (modes of replica r at
 // start[r] ..) into nbuf: the row order of C ([kx][ky][kz], kz fastest), or
 // with plane = 1 that of the plane path's P ([kz][kx][ky], ky fastest).
-template <typename T>
+// WT: FP32 values stored at device scope (the plane path's store shape).
+template <typename T, bool WT = false>
 __device__ __forceinline__ void noise_mode_of(const RepDev &R, long long m, int plane) {
     if (!R.noise) return;
     const int ny = R.n[1], nz = R.n[2];
@@ -118,7 +119,7 @@ __device__ __forceinline__ void noise_mode_of(const RepDev &R, long long m, int 
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         if constexpr (sizeof(T) == 4) {
-            if (plane) stc_dev(nb + d * R.Nc + m, gv[d]);  // the plane path's forcing
+            if (WT) stc_dev(nb + d * R.Nc + m, gv[d]);
             else stc(nb + d * R.Nc + m, gv[d]);
         } else {
             stc(nb + d * R.Nc + m, gv[d]);
@@ -131,7 +132,7 @@ __device__ __forceinline__ void noise_mode_of(const RepDev &R, long long m, int 
 // wavenumber-table values of all K modes are loaded before the first
 // store: stores count in the same vector memory counter as loads, so a
 // load issued after them would also wait for their completion.
-template <typename T, int K>
+template <typename T, int K, bool WT = false>
 __device__ __forceinline__ void noise_modes_rep(const RepDev &R, long long m0, long long stride, long long tot,
                                                 int plane) {
     if (!R.noise) return;
@@ -167,7 +168,7 @@ __device__ __forceinline__ void noise_modes_rep(const RepDev &R, long long m0, l
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             if constexpr (sizeof(T) == 4) {
-                if (plane) stc_dev(nb + d * R.Nc + m, gv[d]);  // the plane path's forcing
+                if (WT) stc_dev(nb + d * R.Nc + m, gv[d]);
                 else stc(nb + d * R.Nc + m, gv[d]);
             } else {
                 stc(nb + d * R.Nc + m, gv[d]);
@@ -179,13 +180,13 @@ __device__ __forceinline__ void noise_modes_rep(const RepDev &R, long long m0, l
 // When the wave's active lanes all draw modes of one replica (always for a
 // single system), the replica index is taken from the first active lane so
 // that the descriptor's fields come through scalar loads.
-template <typename T>
+template <typename T, bool WT = false>
 __device__ __forceinline__ void noise_mode(const RepDev *__restrict__ reps, int nrep,
                                            const long long *__restrict__ start, long long g, int plane) {
     const int r = find_rep(start, nrep, g);
     const int r0 = __builtin_amdgcn_readfirstlane(r);
-    if (__ballot(r != r0) == 0) noise_mode_of<T>(reps[r0], g - start[r0], plane);
-    else noise_mode_of<T>(reps[r], g - start[r], plane);
+    if (__ballot(r != r0) == 0) noise_mode_of<T, WT>(reps[r0], g - start[r0], plane);
+    else noise_mode_of<T, WT>(reps[r], g - start[r], plane);
 }
 
 }  // namespace selm

@@ -361,9 +361,9 @@ struct SpreadShared {
 // Tile t with cnt bucket insertions this step: the scale from every impulse
 // of the tile, then the bucket entries' units and the tile's overflow pairs
 // window by window, all adding into one LDS accumulator; the tile is
-// rounded to T and written to the pool.  Every thread runs the same
-// sequence of barriers.
-template <typename T>
+// rounded to T and written to the pool (WT: at device scope, FP32 only).
+// Every thread runs the same sequence of barriers.
+template <typename T, bool WT>
 __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uint32_t step, int par,
                                             SpreadShared &sh) {
     const int tid = threadIdx.x;
@@ -430,7 +430,7 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     for (int q = tid; q < 3 * tcells; q += kSpreadThreads) {
         const int dd = q / tcells;
         const T v = (T)((double)(long long)sh.acc[dd * kTileCells + (q - dd * tcells)] * inv);
-        if constexpr (sizeof(T) == 4) st_dev(pool + q, v);  // FP32: device-scope, as the plane path's outputs
+        if constexpr (WT && sizeof(T) == 4) st_dev(pool + q, v);
         else pool[q] = v;
     }
     if (tid == 0) {
@@ -452,7 +452,9 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
 // tile, or with R.split_at > 0 those of at most split_at bucket entries.
 // Waves per SIMD: 7 in FP32-bulk (at most 72 VGPRs, a few spilled; 1 µs
 // faster per step at 128^3, 20k beads, than 6 with none spilled), 5 in FP64.
-template <typename T>
+// WT: the batch's store shape (FP32-bulk plane path): forcing and pool
+// stored at device scope.
+template <typename T, bool WT>
 __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 7 : 5))) void k_spread(
     const RepDev *__restrict__ reps, int nrep,
                                                            const long long *__restrict__ st_modes,
@@ -469,10 +471,10 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
                 // one system: four modes per call, their table values loaded
                 // before the first store
                 for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes; m += 4 * stride)
-                    noise_modes_rep<T, 4>(reps[0], m, stride, tot_modes, 1);
+                    noise_modes_rep<T, 4, WT>(reps[0], m, stride, tot_modes, 1);
             } else {
                 for (long long m = (long long)blockIdx.x * kSpreadThreads + threadIdx.x; m < tot_modes; m += stride)
-                    noise_mode<T>(reps, nrep, st_modes, m, 1);
+                    noise_mode<T, WT>(reps, nrep, st_modes, m, 1);
             }
             return;
         }
@@ -490,7 +492,7 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
         __builtin_amdgcn_s_setprio(3);
         for (int k = j; k < nsplit; k += G) {
             const int t = gbl(R.slist)[k];
-            spread_tile<T>(R, t, R.bcnt[t], step, par, sh);
+            spread_tile<T, WT>(R, t, R.bcnt[t], step, par, sh);
         }
         return;
     }
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
     if (cnt == 0 || (R.split_at > 0 && cnt > R.split_at)) return;
     // issue priority over the thermal forcing
     __builtin_amdgcn_s_setprio(3);
-    spread_tile<T>(R, t, cnt, step, par, sh);
+    spread_tile<T, WT>(R, t, cnt, step, par, sh);
 }
 
 // u* over the whole grid into fsp (runtime-length FFT path only)
@@ -1074,7 +1076,9 @@ static void launch_stage_t(const StepLaunch &L, int stage, hipStream_t st) {
             // the plane path's thermal forcing (FP32-bulk only)
             const int nnoise =
                 L.plane && L.tot_modes > 0 ? (L.noise_blocks > 0 ? L.noise_blocks : spread_noise_default(L.tot_modes)) : 0;
-            hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)(nnoise + L.tot_split + L.tot_tile)),
+            // device-scope stores only with the plane path (FP32-bulk)
+            auto *kern = (sizeof(T) == 4 && L.plane && L.wt) ? k_spread<T, sizeof(T) == 4> : k_spread<T, false>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)(nnoise + L.tot_split + L.tot_tile)),
                                dim3(kSpreadThreads), 0, st, reps, nrep, L.st_modes, L.tot_modes, nnoise, L.st_split,
                                L.tot_split, L.st_tile, L.tot_tile);
             if (!L.fast)

@@ -34,6 +34,9 @@ struct StepLaunch {
     bool lj_bins;                    // some replica bins its beads for lj/cut
     int lj_maxbins;                  // the most lj/cut bins of one replica
     bool plane;                      // ... as the three-pass plane path (XYf, Zq, XYi)
+    bool wt;                         // plane path: bulk outputs (forcing, spread pool, the three
+                                     // passes' results) stored at device scope, written through
+                                     // the XCD's L2 (the batch's store shape)
     int okmax;                       // plane path: LDS ints for one plane's tile stamps
     int zq_cols;                     // plane path: (kx, ky) columns per Z-pass workgroup (17 or 9)
     bool aux_bonds;                  // plane path: the XYi launch evaluates the bond forces (no
