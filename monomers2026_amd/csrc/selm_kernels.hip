@@ -91,6 +91,19 @@ constexpr int kBeadsPerBlock = kBeadThreads / 16;
 
 // Bead b of replica R, by the 16 lanes of its group (active: a real bead;
 // inactive groups mirror the last bead and store nothing).
+//
+// The order of the memory operations follows from the vector memory counter
+// (vmcnt), which counts loads, stores and atomics alike and lets a wait for
+// one of them wait for every older one too:
+//  1. the bead's position and state (x, v, f_cons, m);
+//  2. the 12 gather loads of the three 4x4x4 stencils;
+//  3. the slot claims in the bucket of every tile the stencils touch
+//     (returning atomics), whose round trip overlaps the gather's;
+//  4. the bead kick's normal, drawn while both are in flight;
+//  5. the state stores, then the bucket entries, whose slots have arrived
+//     by then.
+// No load waits for a store, and the claims' results wait only for the
+// loads issued before them.
 template <typename T>
 __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active) {
     const int sl = threadIdx.x & 15;  // sub-lane: (li, lj) column of the stencil
@@ -102,13 +115,17 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
     double xr[3], s[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) xr[d] = gx[3 * b + d];
+    // the state of component sl (lanes 0..2 of the bead's group use it)
+    const int ist = 3 * b + (sl < 3 ? sl : 0);
+    const double vi = gbl(R.v)[ist], fci = gbl(R.fc)[ist], mb = gbl(R.m)[b];
 #pragma unroll
     for (int d = 0; d < 3; d++) s[d] = (xr[d] - R.lo) / R.dx;
     const uint32_t step = (uint32_t)(R.ts[0] + 1);
 
+    // Gamma: the gather loads of all three components, then their weights
     const int li = sl & 3, lj = sl >> 2;
     const SELM_GLOBAL T *u = gbl((const T *)R.u);
-    double gam[3];
+    T uv[3][4], wk[3][4];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         // component d is sampled at its face centre: the other two axes shift by -1/2
@@ -120,25 +137,54 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
         const T fx = pick4(wx, li), fy = pick4(wy, lj);
         const long long col = (long long)wrap_near(p1 + lj, ny) * nx + wrap_near(p0 + li, nx);
         const SELM_GLOBAL T *ud = u + d * N;
-        T uv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) uv[k] = ud[(long long)wrap_near(p2 + k, nz) * ny * nx + col];
+        for (int k = 0; k < 4; k++) uv[d][k] = ud[(long long)wrap_near(p2 + k, nz) * ny * nx + col];
+#pragma unroll
+        for (int k = 0; k < 4; k++) wk[d][k] = (((T)1 * fz[k]) * fy) * fx;
+    }
+
+    // Slot claims: the bead goes into the bucket of every tile its stencils
+    // touch, the bead's 16 lanes one (tile) insertion each, at most two per
+    // lane (27 tiles at most).
+    int tx[6], ty[6], tz[6];
+    const int cx = axis_tiles(s[0], nx, R.te[0], tx);
+    const int cy = axis_tiles(s[1], ny, R.te[1], ty);
+    const int cz = axis_tiles(s[2], nz, R.te[2], tz);
+    const int ncomb = active ? cx * cy * cz : 0;
+    const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
+    int tq[2], sq[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int q = sl + 16 * h;
+        tq[h] = 0;
+        sq[h] = 0;
+        if (q < ncomb) {
+            const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
+            const int a = q - qa * cx, qb = qa - c * cy;
+            tq[h] = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
+            sq[h] = __hip_atomic_fetch_add(gbl(R.bcnt) + tq[h], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+
+    // the bead kick's normal (lanes 0..2: component sl)
+    double z0 = 0.0;
+    if (active && sl < 3 && R.noise) {
+        double z1;
+        normal_pair_f64(R.seed, kStreamBead, step, (uint32_t)(3 * b + sl), 0u, z0, z1);
+    }
+
+    double gam[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
         T acc = (T)0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc += uv[k] * ((((T)1 * fz[k]) * fy) * fx);
+        for (int k = 0; k < 4; k++) acc += uv[d][k] * wk[d][k];
 #pragma unroll
         for (int m = 1; m < 16; m <<= 1) acc += __shfl_xor(acc, m, 64);
         gam[d] = (double)acc;
     }
     if (!active) return;
 
-    // The state of component sl (lanes 0..2 of the bead's group) is loaded
-    // before any of the bead's stores.  Stores count in the same vector
-    // memory counter as loads, so a load issued after them would wait for
-    // their completion as well (the compiler cannot move these loads up:
-    // the descriptor's pointers might alias).
-    const int ist = 3 * b + (sl < 3 ? sl : 0);
-    const double vi = gbl(R.v)[ist], fci = gbl(R.fc)[ist], mb = gbl(R.m)[b];
     double tfl = 0.0;  // lanes 0..2 of the bead's group: t_f of component sl
     if (sl < 3) {
         const int d = sl;
@@ -147,11 +193,7 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
         const int i = 3 * b + d;
         const double dt = R.dt;
         double tf = -R.ups * (vi - gd) * dt;
-        if (R.noise) {
-            double z0, z1;
-            normal_pair_f64(R.seed, kStreamBead, step, (uint32_t)i, 0u, z0, z1);
-            tf += z0 * R.bead_coef;
-        }
+        if (R.noise) tf += z0 * R.bead_coef;
         tfl = tf;
         gbl(R.tf)[i] = tf;
         gbl(R.sg)[i] = sd;
@@ -170,32 +212,9 @@ __device__ __forceinline__ void bead_update(const RepDev &R, int b, bool active)
         gbl(R.x)[i] = tmp;
         gbl(R.v)[i] = vn;
     }
-    // Insert the bead into the bucket of every tile its stencils touch, the
-    // bead's 16 lanes one (tile) insertion each.  A bucket entry carries what
-    // the spread needs of the bead: s at x^n and t_f.
+    // The bucket entries: what the spread needs of the bead, s at x^n and t_f.
     const int g16 = (threadIdx.x & 63) & ~15;
     const double t0 = __shfl(tfl, g16, 64), t1 = __shfl(tfl, g16 + 1, 64), t2 = __shfl(tfl, g16 + 2, 64);
-    int tx[6], ty[6], tz[6];
-    const int cx = axis_tiles(s[0], nx, R.te[0], tx);
-    const int cy = axis_tiles(s[1], ny, R.te[1], ty);
-    const int cz = axis_tiles(s[2], nz, R.te[2], tz);
-    const int ncomb = cx * cy * cz;  // at most 27: each lane takes at most 2 insertions
-    const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
-    // both of a lane's slot claims are issued before any entry is stored, so
-    // neither waits on the other's stores
-    int tq[2], sq[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int q = sl + 16 * h;
-        tq[h] = 0;
-        sq[h] = 0;
-        if (q < ncomb) {
-            const int qa = div_small(q, cx, rcx), c = div_small(qa, cy, rcy);
-            const int a = q - qa * cx, qb = qa - c * cy;
-            tq[h] = (tz[c] * R.tn[1] + ty[qb]) * R.tn[0] + tx[a];
-            sq[h] = __hip_atomic_fetch_add(gbl(R.bcnt) + tq[h], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         if (sl + 16 * h >= ncomb) break;

@@ -82,17 +82,21 @@ def test_plane_z_shapes_agree(gpu_lib, monkeypatch):
 def test_spread_shapes_give_identical_bits(gpu_lib, monkeypatch):
     """The spread shapes the autotuner chooses between -- split threshold
     64, 32 or none, and the default, 128 or 1024 thermal-forcing workgroups
-    in the spread launch -- give bitwise identical steps: the spread sums are
-    order-independent fixed-point sums and each forcing mode is drawn by one
-    thread whatever the grid.  4000 beads on 64^3 (split list active)."""
+    in the spread launch, and the plane path's two store shapes (device-scope
+    or plain stores of the bulk outputs) -- give bitwise identical steps: the
+    spread sums are order-independent fixed-point sums, each forcing mode is
+    drawn by one thread whatever the grid, and a store's scope changes only
+    where the value is cached.  4000 beads on 64^3 (split list active)."""
     from monomers2026_amd import selm, workloads
     rng = np.random.default_rng(31)
     x, ij = workloads.chains(rng, 40, 100, 64)
     w = workloads.Workload("dense4000_64", 64, x, ij)
     ref = None
-    for sa, wg in (("64", ""), ("32", ""), ("0", ""), ("64", "128"), ("128", "1024")):
+    for sa, wg, wt in (("64", "", "1"), ("32", "", "1"), ("0", "", "1"), ("64", "128", "1"),
+                       ("128", "1024", "1"), ("64", "", "0"), ("0", "1024", "0")):
         monkeypatch.setenv("SELM_SPLIT_AT", sa)
         monkeypatch.setenv("SELM_FORCING_WG", wg)
+        monkeypatch.setenv("SELM_WT_STORES", wt)
         s = w.build_system(selm.PREC_FP32_BULK, True, seed=9)
         shape = s.spread_shape()
         assert shape[0] == int(sa)
@@ -101,7 +105,7 @@ def test_spread_shapes_give_identical_bits(gpu_lib, monkeypatch):
         got = _state(s)
         assert s.status() == 0
         s.close()
-        print("spread shape", shape, flush=True)
+        print("spread shape", shape, "device-scope stores", wt, flush=True)
         if ref is None:
             ref = got
         else:
@@ -109,6 +113,7 @@ def test_spread_shapes_give_identical_bits(gpu_lib, monkeypatch):
                 assert np.array_equal(a, b), shape
     monkeypatch.delenv("SELM_SPLIT_AT", raising=False)
     monkeypatch.delenv("SELM_FORCING_WG", raising=False)
+    monkeypatch.delenv("SELM_WT_STORES", raising=False)
 
 
 def test_autotune_reports_spread_shape(gpu_lib):
