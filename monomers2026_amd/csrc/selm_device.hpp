@@ -90,14 +90,15 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
     // each 32 x 32 -> 64-bit product is one v_mad_u64_u32 (both halves at
-    // once) instead of a v_mul_lo_u32 and a v_mul_hi_u32
+    // once) instead of a v_mul_lo_u32 and a v_mul_hi_u32, and each
+    // three-way XOR one v_bitop3_b32 (truth table 0x96 = a ^ b ^ c)
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
         uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
         uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-        uint32_t n0 = hi1 ^ c[1] ^ k0;
-        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        uint32_t n0 = __builtin_amdgcn_bitop3_b32(hi1, c[1], k0, 0x96);
+        uint32_t n2 = __builtin_amdgcn_bitop3_b32(hi0, c[3], k1, 0x96);
         c[0] = n0;
         c[1] = lo1;
         c[2] = n2;
