@@ -686,17 +686,51 @@ struct ZqShape<9> {
     static constexpr int NT = 256;
 };
 
+// Dynamic LDS: the z twiddles (twmax slots), the Q columns of the three
+// components (a, (3Q+1) N elements), then the wavenumber tables of the
+// workgroup's modes (zt, 5Q + 3N doubles): per column q {lam_x + lam_y,
+// Dre_x, Dim_x, Dre_y, Dim_y}, per kz {lam_z, Dre_z, Dim_z}.  The twiddles
+// and tables are loaded before the columns and written to LDS after the
+// column loads are issued, so that no load waits for another.
 template <typename T, int N, int Q, bool WT>
-__device__ __forceinline__ void zq_body(const RepDev &R, long long item, const cplx<C> *tw,
+__device__ __forceinline__ void zq_body(const RepDev &R, long long item, cplx<C> *tw,
                                         cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     constexpr int S = 3 * Q + 1, NT = ZqShape<Q>::NT;
     constexpr int IP = (N * Q + NT - 1) / NT;  // projection elements per thread
+    constexpr int NTAB = 5 * Q + 3 * N, TWI = (N + NT - 1) / NT, TBI = (NTAB + NT - 1) / NT;
     const int ny = R.n[1], xc = R.xc;
     const int pl = xc * ny;  // columns of one (d, z) plane of P
     const int c0 = (int)item * Q;
     const int cv = min(Q, pl - c0);
     SELM_GLOBAL cplx<T> *Ps = gbl((cplx<T> *)R.spec) + c0;
+    C *zt = (C *)(a + S * N);
+    const float rny = 1.0f / (float)ny;
+    cplx<C> twv[TWI];
+    {
+        const SELM_GLOBAL cplx<C> *tws = gbl((const cplx<C> *)R.twd[2]);
+#pragma unroll
+        for (int k = 0; k < TWI; k++) twv[k] = ldc(tws + min((int)threadIdx.x + k * NT, N - 1));
+    }
+    C tbv[TBI];
+    {
+        const SELM_GLOBAL C *kxt = gbl((const C *)R.ktabd[0]), *kyt = gbl((const C *)R.ktabd[1]),
+                           *kzt = gbl((const C *)R.ktabd[2]);
+#pragma unroll
+        for (int k = 0; k < TBI; k++) {
+            const int v = min((int)threadIdx.x + k * NT, NTAB - 1);
+            if (v < 5 * Q) {
+                const int q = v / 5, f = v - 5 * q;
+                const int c = c0 + min(q, cv - 1);
+                const int i = div_small(c, ny, rny), j = c - i * ny;
+                if (f == 0) tbv[k] = kxt[4 * i] + kyt[4 * j];
+                else tbv[k] = f < 3 ? kxt[4 * i + f] : kyt[4 * j + f - 2];
+            } else {
+                const int w = v - 5 * Q, kz = w / 3;
+                tbv[k] = kzt[4 * kz + (w - 3 * kz)];
+            }
+        }
+    }
     // sequence s = d*Q + q <-> column c0+q of component d; element e = z
     copy_batched<NT, 3 * Q * N, 16>(
         [&](int idx) {
@@ -709,6 +743,12 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
             const int d = dz / N, e = dz - d * N;
             a[e * S + d * Q + q] = ccast<L>(v);
         });
+#pragma unroll
+    for (int k = 0; k < TWI; k++)
+        if ((int)threadIdx.x + k * NT < N) tw[threadIdx.x + k * NT] = twv[k];
+#pragma unroll
+    for (int k = 0; k < TBI; k++)
+        if ((int)threadIdx.x + k * NT < NTAB) zt[threadIdx.x + k * NT] = tbv[k];
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
     auto nost = [](int, int, cplx<C>) {};
@@ -727,26 +767,24 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
         }
     }
 
-    const C *kxt = (const C *)R.ktabd[0], *kyt = (const C *)R.ktabd[1], *kzt = (const C *)R.ktabd[2];
     const C invN = 1.0 / (double)R.N;
-    const float rny = 1.0f / (float)ny;
 #pragma unroll
     for (int kk = 0; kk < IP; kk++) {
         const int idx = (int)threadIdx.x + kk * NT;
         if (idx >= N * Q) continue;
         const int q = idx % Q, e = idx / Q;
         if (q >= cv) continue;
-        const int c = c0 + q;
-        const int i = div_small(c, ny, rny), j = c - i * ny, k = e;
+        const int c = c0 + q, k = e;
         cplx<C> w[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) w[d] = ccast<C>(a[e * S + d * Q + q]);
-        const C lam = (kxt[4 * i] + kyt[4 * j]) + kzt[4 * k];
+        const C *zq = zt + 5 * q, *zk = zt + 5 * Q + 3 * k;
+        const C lam = zq[0] + zk[0];
         if (R.noise) {
 #pragma unroll
             for (int d = 0; d < 3; d++) w[d] = cadd(w[d], ccast<C>(nv[kk][d]));
         }
-        if (i == 0 && j == 0 && k == 0) {
+        if (c == 0 && k == 0) {
             if (R.diag) {
 #pragma unroll
                 for (int d = 0; d < 3; d++) R.diag[d] = (double)w[d].x;
@@ -755,9 +793,9 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, const c
             for (int d = 0; d < 3; d++) a[e * S + d * Q + q] = ccast<L>(czero<C>());
             continue;
         }
-        const cplx<C> D0 = mk<C>(kxt[4 * i + 1], kxt[4 * i + 2]);
-        const cplx<C> D1 = mk<C>(kyt[4 * j + 1], kyt[4 * j + 2]);
-        const cplx<C> D2 = mk<C>(kzt[4 * k + 1], kzt[4 * k + 2]);
+        const cplx<C> D0 = mk<C>(zq[1], zq[2]);
+        const cplx<C> D1 = mk<C>(zq[3], zq[4]);
+        const cplx<C> D2 = mk<C>(zk[1], zk[2]);
         const cplx<C> sdiv = cadd(cadd(cmul(D0, w[0]), cmul(D1, w[1])), cmul(D2, w[2]));
         const cplx<C> qv = mk<C>(sdiv.x / lam, sdiv.y / lam);
         const cplx<C> G[3] = {mk<C>(-D0.x, D0.y), mk<C>(-D1.x, D1.y), mk<C>(-D2.x, D2.y)};
@@ -937,8 +975,27 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
 
 SELM_FFT_KERNEL(k_xf, 0, SELM_CASE_XF, NT)
 // 4 waves per SIMD: two 512-thread workgroups per CU
-SELM_FFT_KERNEL(k_zq, 2, SELM_CASE_ZQ, NTZ, __attribute__((amdgpu_waves_per_eu(4))))
-SELM_FFT_KERNEL(k_zq9, 2, SELM_CASE_ZQ9, NTZ9, __attribute__((amdgpu_waves_per_eu(4))))
+// The Z pass of the plane path fills its own twiddles (after its first loads).
+#define SELM_ZQ_KERNEL(kname, BODY, NTH)                                                              \
+    template <typename T, int NMAX, bool WT = false>                                                  \
+    __global__ __launch_bounds__(FastCfg<T>::NTH) __attribute__((amdgpu_waves_per_eu(4)))             \
+    void kname(const RepDev *__restrict__ reps, int nrep, const long long *__restrict__ start,        \
+               long long total, int twmax) {                                                          \
+        extern __shared__ __align__(16) unsigned char smem[];                                         \
+        const long long g = xcd_remap(blockIdx.x, gridDim.x);                                         \
+        if (g >= total) return;                                                                       \
+        const int r = find_rep(start, nrep, g);                                                       \
+        const RepDev &R = reps[r];                                                                    \
+        const long long item = g - start[r];                                                          \
+        cplx<C> *tw = (cplx<C> *)smem;                                                                \
+        auto *a = (cplx<typename FastCfg<T>::L> *)(tw + twmax);                                       \
+        switch (R.n[2]) {                                                                             \
+            SELM_FFT_FOR_EACH_N(BODY)                                                                 \
+            default: break;                                                                           \
+        }                                                                                             \
+    }
+SELM_ZQ_KERNEL(k_zq, SELM_CASE_ZQ, NTZ)
+SELM_ZQ_KERNEL(k_zq9, SELM_CASE_ZQ9, NTZ9)
 
 // XY passes of the plane path.  Dynamic LDS: the x and y twiddles (twmax
 // slots each), the tile stamps of one plane (okmax ints), then the plane.
@@ -1055,7 +1112,8 @@ void fft_plane_lds(bool fp64, const int n[3], int twmax, int okmax, int zcols, s
     // twiddles x and y, the row and column pool tables, the tile stamps, the plane
     *lxy = 2 * (size_t)twmax * sizeof(cplx<C>) + 2 * (size_t)twmax * 8 + (size_t)okmax * 4 +
            std::max(N * (P + 1) * csz, N * N * tsz);
-    *lz = (size_t)twmax * sizeof(cplx<C>) + (size_t)(3 * Q + 1) * n[2] * csz;
+    // z twiddles, the columns, the wavenumber tables of the workgroup's modes
+    *lz = (size_t)twmax * sizeof(cplx<C>) + (size_t)(3 * Q + 1) * n[2] * csz + (size_t)(5 * Q + 3 * n[2]) * 8;
 }
 
 // workgroups of the XY passes and of the Z pass for one replica
