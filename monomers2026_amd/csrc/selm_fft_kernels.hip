@@ -50,6 +50,25 @@ struct FastCfg<double> {
     static constexpr int NT = 256, NTZ = 512, NTZ9 = 256;
 };
 
+// Twiddles of a length-N axis, loaded into registers (TWI = ceil(N / NT) per
+// thread) so that a kernel can issue its data loads before it writes them to
+// LDS: a write to LDS right after the load would wait for the load.
+template <int N, int NT>
+struct TwRegs {
+    static constexpr int TWI = (N + NT - 1) / NT;
+    cplx<C> v[TWI];
+    __device__ __forceinline__ void load(const void *src) {
+        const SELM_GLOBAL cplx<C> *s = gbl((const cplx<C> *)src);
+#pragma unroll
+        for (int k = 0; k < TWI; k++) v[k] = ldc(s + min((int)threadIdx.x + k * NT, N - 1));
+    }
+    __device__ __forceinline__ void store(cplx<C> *dst) const {
+#pragma unroll
+        for (int k = 0; k < TWI; k++)
+            if ((int)threadIdx.x + k * NT < N) dst[threadIdx.x + k * NT] = v[k];
+    }
+};
+
 __device__ __forceinline__ void tw_to_lds(cplx<C> *dst, const void *src, int n) {
     const SELM_GLOBAL cplx<C> *s = gbl((const cplx<C> *)src);
     for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = ldc(s + m);
@@ -457,8 +476,8 @@ __device__ __forceinline__ void lds_remap(const LD &ld, const ST &st) {
 }
 
 template <typename T, int N, int NT, bool WT>
-__device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const cplx<C> *twx,
-                                         const cplx<C> *twy, int *okf, int2 *tab, cplx<typename FastCfg<T>::L> *a) {
+__device__ __forceinline__ void xyf_body(const RepDev &R, long long item, cplx<C> *twx,
+                                         cplx<C> *twy, int *okf, int2 *tab, cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     static_assert(sizeof(T) == 4 && sizeof(L) == 4, "the plane path is FP32-bulk");
     constexpr int P = N / 2, S = P + 1, NN = N * N, IT = (NN + NT - 1) / NT;
@@ -468,14 +487,24 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
     const SELM_GLOBAL T *ud = gbl((const T *)R.u) + d * R.N;
     T *U = (T *)a;  // plane z of u_d, [y][x]
 
-    // stamps of the tile row under plane z, and the pool addressing split
-    // into a row part and a column part: the spread value of cell (x, y) is
-    // pool[row(y).x + col(x).x], valid when okf[row(y).y + col(x).y]
+    // The x and y twiddles and the stamps of the tile row under plane z are
+    // loaded first and written to LDS after the plane's loads are issued, so
+    // that no load waits for another.  The pool addressing is split into a
+    // row part and a column part: the spread value of cell (x, y) is
+    // pool[row(y).x + col(x).x], valid when okf[row(y).y + col(x).y].
+    TwRegs<N, NT> twxr, twyr;
+    twxr.load(R.twd[0]);
+    twyr.load(R.twd[1]);
     const int te0 = R.te[0], te1 = R.te[1], te2 = R.te[2], tn0 = R.tn[0], tn1 = R.tn[1];
     const int tz = z / te2;
     const int tcells = te0 * te1 * te2;
+    const int ntr = tn0 * tn1;
     const SELM_GLOBAL int *stamp = gbl((const int *)R.tstamp) + tz * tn0 * tn1;
-    for (int i = threadIdx.x; i < tn0 * tn1; i += NT) okf[i] = stamp[i] == step;
+    // tiles of at least 8 cells per edge: at most OKI stamps per thread
+    constexpr int OKI = (((N + 7) / 8) * ((N + 7) / 8) + NT - 1) / NT;
+    int okv[OKI];
+#pragma unroll
+    for (int k = 0; k < OKI; k++) okv[k] = stamp[min((int)threadIdx.x + k * NT, ntr - 1)];
     int2 *rowt = tab, *colt = tab + N;
     for (int i = threadIdx.x; i < N; i += NT) {
         const int ty = i / te1, tx = i / te0;
@@ -495,6 +524,12 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, const 
         copy_batched<NT, NQ, (IQ < 16 ? IQ : 16)>([&](int idx) -> f32x4 { return uz4[idx]; },
                                                    [&](int idx, f32x4 v) { U4[idx] = v; });
     }
+    twxr.store(twx);
+    twyr.store(twy);
+#pragma unroll
+    for (int k = 0; k < OKI; k++)
+        if ((int)threadIdx.x + k * NT < ntr) okf[threadIdx.x + k * NT] = okv[k] == step;
+    for (int i = threadIdx.x + OKI * NT; i < ntr; i += NT) okf[i] = stamp[i] == step;  // smaller tiles
     __syncthreads();
 
     // u* (the arithmetic of ustar_at, cuda/SELM_Kernel.cu:155-195) of this
@@ -815,14 +850,19 @@ __device__ __forceinline__ void zq_body(const RepDev &R, long long item, cplx<C>
 }
 
 template <typename T, int N, int NT, bool WT>
-__device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const cplx<C> *twx,
-                                         const cplx<C> *twy, cplx<typename FastCfg<T>::L> *a) {
+__device__ __forceinline__ void xyi_body(const RepDev &R, long long item, cplx<C> *twx,
+                                         cplx<C> *twy, cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     constexpr int P = N / 2, S = P + 1, NN = N * N;
     const int nz = R.n[2];
     const int z = (int)(item % nz), d = (int)(item / nz);
     const SELM_GLOBAL cplx<T> *in = gbl((const cplx<T> *)R.spec) + ((long long)d * nz + z) * (P + 1) * N;  // [kx][ky]
     constexpr int NI = (P - 1) * N, ITI = (NI + NT - 1) / NT, I0 = (N + NT - 1) / NT;
+    // the twiddles are loaded first and written to LDS after the spectrum's
+    // loads are issued
+    TwRegs<N, NT> twxr, twyr;
+    twxr.load(R.twd[0]);
+    twyr.load(R.twd[1]);
     // sequence 0 = Y0h + i YPh, the Hermitian parts of the kx = 0 and kx = P
     // columns (what a c2r keeps of them), whose inverses are real; its loads
     // are issued first and complete under the batched copy below
@@ -851,6 +891,8 @@ __device__ __forceinline__ void xyi_body(const RepDev &R, long long item, const 
             a[ky * S] = mk<L>(A0.x - B0.y, A0.y + B0.x);
         }
     }
+    twxr.store(twx);
+    twyr.store(twy);
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
     auto nost = [](int, int, cplx<C>) {};
@@ -998,7 +1040,9 @@ SELM_ZQ_KERNEL(k_zq, SELM_CASE_ZQ, NTZ)
 SELM_ZQ_KERNEL(k_zq9, SELM_CASE_ZQ9, NTZ9)
 
 // XY passes of the plane path.  Dynamic LDS: the x and y twiddles (twmax
-// slots each), the tile stamps of one plane (okmax ints), then the plane.
+// slots each), the pool tables, the tile stamps of one plane (okmax ints),
+// then the plane.  The bodies write the twiddles and stamps after issuing
+// the plane's loads.
 // Auxiliary workgroups (plane_block_role) stride over aux_total work items
 // (AUX(i) for each) with aux * NT threads.
 #define SELM_PLANE_KERNEL(kname, BODY, AUX)                                                           \
@@ -1023,8 +1067,6 @@ SELM_ZQ_KERNEL(k_zq9, SELM_CASE_ZQ9, NTZ9)
         int2 *tab = (int2 *)(twy + twmax);                                                            \
         int *okf = (int *)(tab + 2 * twmax);                                                          \
         auto *a = (cplx<typename FastCfg<T>::L> *)(okf + okmax);                                      \
-        tw_to_lds(twx, R.twd[0], R.n[0]);                                                             \
-        tw_to_lds(twy, R.twd[1], R.n[1]);                                                             \
         (void)okf;                                                                                    \
         (void)tab;                                                                                    \
         switch (R.n[0]) {                                                                             \
