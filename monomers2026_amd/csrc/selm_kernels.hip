@@ -401,11 +401,25 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     if (tid == 0) sh.emax = 0;
     for (int q = tid; q < 3 * kTileCells; q += kSpreadThreads) sh.acc[q] = 0ull;
     // the scale: the largest exponent of every impulse of the tile (bucket
-    // entries' pieces 1 and 2, and the overflow pairs)
-    int e = 0;
-    for (int i = tid; i < nb; i += kSpreadThreads) {
-        const f64x2 e1 = src[3 * i + 1], e2 = src[3 * i + 2];
-        e = max(e, max(exp_bits(e1.y), max(exp_bits(e2.x), exp_bits(e2.y))));
+    // entries' pieces 1 and 2, and the overflow pairs).  Thread tid takes the
+    // impulses of units tid, tid + 256, ... (unit u = component u % 3 of
+    // entry u / 3), as in the unit pass below, and keeps the entry of its
+    // first unit in registers for that pass: one load of the entries for
+    // tiles of up to 85.
+    const bool has0 = tid < 3 * nb;
+    const int k0 = tid / 3, d0 = tid - 3 * k0;
+    f64x2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
+    if (has0) {
+        a0 = src[3 * k0];
+        a1 = src[3 * k0 + 1];
+        a2 = src[3 * k0 + 2];
+    }
+    const double tf0 = d0 == 0 ? a1.y : (d0 == 1 ? a2.x : a2.y);
+    int e = has0 ? exp_bits(tf0) : 0;
+    for (int u = tid + kSpreadThreads; u < 3 * nb; u += kSpreadThreads) {
+        const int k = u / 3, dd = u - 3 * k;
+        const f64x2 e1 = src[3 * k + 1], e2 = src[3 * k + 2];
+        e = max(e, exp_bits(dd == 0 ? e1.y : (dd == 1 ? e2.x : e2.y)));
     }
     for (int o = tid; o < novf; o += kSpreadThreads)
         if (R.ovf[2 * o] == t) {
@@ -422,7 +436,11 @@ __device__ __forceinline__ void spread_tile(const RepDev &R, int t, int cnt, uin
     __syncthreads();
     const int E = sh.emax;
     const double scale = ldexp(1.0, spread_shift(E));
-    for (int u = tid; u < 3 * nb; u += kSpreadThreads) {
+    if (has0) {
+        const double sg[3] = {a0.x, a0.y, a1.x};
+        spread_unit<T>(accb, sg, tf0, d0, f4, tl, scale);
+    }
+    for (int u = tid + kSpreadThreads; u < 3 * nb; u += kSpreadThreads) {
         const int k = u / 3, dd = u - 3 * k;
         const f64x2 e0 = src[3 * k], e1 = src[3 * k + 1], e2 = src[3 * k + 2];
         const double sg[3] = {e0.x, e0.y, e1.x};
