@@ -169,7 +169,8 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
             XfLoads<T> v;
             v.zp = up[row];
             v.zm = um[row];
-            v.f = pool[(t * 3 + d) * tcells + lyz + (e - tx * te0)];
+            // only the tiles a bead touched this step are read
+            v.f = okf[(y / te1 - ty0) * FW + tx] ? pool[(t * 3 + d) * tcells + lyz + (e - tx * te0)] : (T)0;
             return v;
         },
         [&](int idx, XfLoads<T> v) {
@@ -183,8 +184,7 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
                 const C lap = ((C)row[e + 1 < N ? e + 1 : 0] + (C)row[e > 0 ? e - 1 : N - 1] +
                                (C)X[(rl + 2) * N + e] + (C)X[rl * N + e] + (C)v.zp + (C)v.zm - (C)6 * uc) /
                               dxsq;
-                const C f = okf[(y / te1 - ty0) * FW + e / te0] ? (C)v.f : (C)0;
-                us = uc + (mu * lap * dt + f) / rho;
+                us = uc + (mu * lap * dt + (C)v.f) / rho;
             }
             ((L *)&a[e * S + (rl >> 1)])[rl & 1] = (L)us;
         });
@@ -562,8 +562,11 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, cplx<C
                 const int2 rt = rowt[y];
                 v[j].zp = up[idx];
                 v[j].zm = um[idx];
-                v[j].f = pool[rt.x + ct.x];
-                ok[j] = rt.y + ct.y;
+                // only the tiles a bead touched this step are read: another
+                // cell's load reads pool element 0 (one cached line), and its
+                // value is replaced by 0 (no branch in the batch)
+                ok[j] = okf[rt.y + ct.y];
+                v[j].f = pool[ok[j] ? rt.x + ct.x : 0];
             }
 #pragma unroll
             for (int j = 0; j < BT; j++) {
@@ -575,8 +578,7 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, cplx<C
                 const float uc = row[x];
                 // neighbour order x+, x-, y+, y-, z+, z- (as ustar_at)
                 const float nb = (((((row[xp] + row[xm]) + uyp) + uym) + v[j].zp) + v[j].zm) - 6.0f * uc;
-                const float f = okf[ok[j]] ? v[j].f : 0.0f;
-                us[k] = uc + c_lap * nb + c_f * f;
+                us[k] = uc + c_lap * nb + c_f * (ok[j] ? v[j].f : 0.0f);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -602,8 +604,9 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, cplx<C
             const int2 rt = rowt[y], ct = colt[x];
             v[j].zp = up[idx];
             v[j].zm = um[idx];
-            v[j].f = pool[rt.x + ct.x];
-            ok[j] = rt.y + ct.y;
+            // only the tiles a bead touched this step are read (as above)
+            ok[j] = okf[rt.y + ct.y];
+            v[j].f = pool[ok[j] ? rt.x + ct.x : 0];
         }
 #pragma unroll
         for (int j = 0; j < BT; j++) {
@@ -619,8 +622,7 @@ __device__ __forceinline__ void xyf_body(const RepDev &R, long long item, cplx<C
                                    v[j].zp) +
                                   v[j].zm) -
                                  6.0f * uc;
-                const float f = okf[ok[j]] ? v[j].f : 0.0f;
-                us[k0 + j] = uc + c_lap * nb + c_f * f;
+                us[k0 + j] = uc + c_lap * nb + c_f * (ok[j] ? v[j].f : 0.0f);
             }
         }
     }
