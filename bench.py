@@ -116,13 +116,13 @@ def reference_stage_bytes(w, s):
 # bond forces.
 PMC_TRAFFIC_FP32 = {
     "thermal_noise": 0,
-    "bead_update": 31788765,
-    "spread": 43183792,
-    "ustar_fft_x": 89830029,
+    "bead_update": 31809740,
+    "spread": 47084664,
+    "ustar_fft_x": 68842014,
     "fft_y": 0,
-    "fft_z_projection": 83413826,
+    "fft_z_projection": 83422274,
     "ifft_y": 0,
-    "ifft_x": 53559864,
+    "ifft_x": 53554104,
     "bond_forces": 0,
 }
 COPY_CEILING_GBS = 5970.0  # measured: torch elementwise copy of 256 MiB in the same counter runs
