@@ -69,10 +69,6 @@ struct TwRegs {
     }
 };
 
-__device__ __forceinline__ void tw_to_lds(cplx<C> *dst, const void *src, int n) {
-    const SELM_GLOBAL cplx<C> *s = gbl((const cplx<C> *)src);
-    for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = ldc(s + m);
-}
 
 template <typename T>
 __device__ __forceinline__ cplx<T> czero() {
@@ -119,6 +115,9 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
                                         cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     constexpr int RB = FastCfg<T>::RB, P = RB / 2, S = P + 1, NT = FastCfg<T>::NT, XC = N / 2 + 1;
+    // the twiddles are written to LDS after the data loads are issued
+    TwRegs<N, NT> twr;
+    twr.load(R.twd[0]);
     const int ny = R.n[1], nz = R.n[2];
     const int nyb = (ny + RB - 1) / RB;
     const int yb = (int)(item % nyb);
@@ -137,9 +136,14 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
     const int tz = z / te2;
     const int yl = min(y0 + RB, ny) - 1;
     const int ty0 = y0 / te1, nty = yl / te1 - ty0 + 1;
-    for (int i = threadIdx.x; i < nty * tn0; i += NT) {
+    // stamps loaded here, written to LDS after the rows' loads are issued
+    constexpr int OKI = (FH * FW + NT - 1) / NT;
+    int okv[OKI];
+#pragma unroll
+    for (int k = 0; k < OKI; k++) {
+        const int i = min((int)threadIdx.x + k * NT, nty * tn0 - 1);
         const int q = i / tn0, tx = i - q * tn0;
-        okf[q * FW + tx] = R.tstamp[(tz * tn1 + ty0 + q) * tn0 + tx] == step;
+        okv[k] = R.tstamp[(tz * tn1 + ty0 + q) * tn0 + tx];
     }
     constexpr int SROWS = (RB + 2) * N;
     copy_batched<NT, SROWS, (SROWS + NT - 1) / NT>(
@@ -151,6 +155,15 @@ __device__ __forceinline__ void xf_body(const RepDev &R, long long item, cplx<C>
             return ud[(long long)z * Nyx + (long long)y * N + e];
         },
         [&](int idx, T v) { X[idx] = v; });
+    twr.store(tw);
+#pragma unroll
+    for (int k = 0; k < OKI; k++) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < nty * tn0) {
+            const int q = i / tn0, tx = i - q * tn0;
+            okf[q * FW + tx] = okv[k] == step;
+        }
+    }
     __syncthreads();
 
     const T *up = ud + (long long)(z + 1 < nz ? z + 1 : 0) * Nyx;
@@ -217,6 +230,9 @@ __device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<C> 
                                        cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     constexpr int RB = FastCfg<T>::RB, S = RB + 1, NT = FastCfg<T>::NT;
+    // the twiddles are written to LDS after the data loads are issued
+    TwRegs<N, NT> twr;
+    twr.load(R.twd[1]);
     const int nz = R.n[2], xc = R.xc;
     const int nzb = (nz + RB - 1) / RB;
     const int zb = (int)(item % nzb);
@@ -249,6 +265,7 @@ __device__ __forceinline__ void y_body(const RepDev &R, long long item, cplx<C> 
                 a[ky * S + q] = ccast<L>(v);
             });
     }
+    twr.store(tw);
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
     auto nost = [](int, int, cplx<C>) {};
@@ -289,6 +306,9 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<C> 
                                        cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     constexpr int Q = FastCfg<T>::Q, S = 3 * Q + 1, NT = FastCfg<T>::NTZ;
+    // the twiddles are written to LDS after the data loads are issued
+    TwRegs<N, NT> twr;
+    twr.load(R.twd[2]);
     const int ny = R.n[1], xc = R.xc;
     const int nyb = (ny + Q - 1) / Q;
     const int jb = (int)(item % nyb);
@@ -308,6 +328,7 @@ __device__ __forceinline__ void z_body(const RepDev &R, long long item, cplx<C> 
             const int s = idx / N, e = idx - s * N;
             a[e * S + s] = ccast<L>(v);
         });
+    twr.store(tw);
     __syncthreads();
     auto nold = [](int, int) { return cplx<C>{}; };
     auto nost = [](int, int, cplx<C>) {};
@@ -367,6 +388,9 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C>
                                         cplx<typename FastCfg<T>::L> *a) {
     using L = typename FastCfg<T>::L;
     constexpr int RB = FastCfg<T>::RB, P = RB / 2, S = P + 1, NT = FastCfg<T>::NT, XC = N / 2 + 1;
+    // the twiddles are written to LDS after the data loads are issued
+    TwRegs<N, NT> twr;
+    twr.load(R.twd[0]);
     constexpr int SS = RB + 1;
     const int ny = R.n[1], nz = R.n[2];
     const int nyb = (ny + RB - 1) / RB;
@@ -386,6 +410,7 @@ __device__ __forceinline__ void xi_body(const RepDev &R, long long item, cplx<C>
             const int k = idx / RB, rl = idx - k * RB;
             stg[k * SS + rl] = ccast<L>(v);
         });
+    twr.store(tw);
     __syncthreads();
     for (int idx = threadIdx.x; idx < P * N; idx += NT) {
         const int p = idx / N, e = idx - p * N;
@@ -994,7 +1019,6 @@ __global__ __launch_bounds__(256) void k_noise(const RepDev *__restrict__ reps, 
         const long long item = g - start[r];                                                          \
         cplx<C> *tw = (cplx<C> *)smem;                                                                \
         auto *a = (cplx<typename FastCfg<T>::L> *)(tw + twmax);                                       \
-        tw_to_lds(tw, R.twd[axis], R.n[axis]);                                                        \
         switch (R.n[axis]) {                                                                          \
             SELM_FFT_FOR_EACH_N(BODY)                                                                 \
             default: break;                                                                           \
