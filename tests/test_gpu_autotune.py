@@ -61,6 +61,32 @@ def test_pool_autotune_restores_state(gpu_lib):
     pool.close()
 
 
+def test_pool_store_shapes_give_identical_bits(gpu_lib, monkeypatch):
+    """A pool of six systems of the ensemble mix (edges 16-64) on the plane
+    path, stepped with the bulk outputs stored at device scope and stored
+    plainly: every system's beads and field agree bit for bit."""
+    from monomers2026_amd import selm, workloads
+    ws = workloads.ensemble(n=6)
+    out = {}
+    for wt in ("1", "0"):
+        monkeypatch.setenv("SELM_WT_STORES", wt)
+        fps = [selm.footprint(w.params(selm.PREC_FP32_BULK, True, seed=70 + i), w.M, w.ij.shape[0])
+               for i, w in enumerate(ws)]
+        pool = selm.Pool(int(sum(fps) * 1.05) + (16 << 20), len(ws), precision=selm.PREC_FP32_BULK)
+        slots = [pool.admit(w.params(selm.PREC_FP32_BULK, True, seed=70 + i), w.x, w.v, w.m, w.ij, w.K, w.r0)
+                 for i, w in enumerate(ws)]
+        pool.step(6)
+        pool.sync()
+        out[wt] = [(pool.beads(sl), pool.field(sl)) for sl in slots]
+        pool.close()
+        print("pool store shape %s: stepped" % wt, flush=True)
+    monkeypatch.delenv("SELM_WT_STORES", raising=False)
+    for (b0, f0), (b1, f1) in zip(out["1"], out["0"]):
+        for a, b in zip(b0, b1):
+            assert np.array_equal(a, b)
+        assert np.array_equal(f0, f1)
+
+
 def test_plane_z_shapes_agree(gpu_lib, monkeypatch):
     from monomers2026_amd import selm, workloads
     w = workloads.chains1000_64()
